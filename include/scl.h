@@ -1,0 +1,174 @@
+/*
+ * scl.h - C ABI of the MI355X pseudo-spectral RHS + IF-RK4 library.
+ *
+ * Every function returns SCL_OK (0) or a negative status; scl_last_error()
+ * gives the message of the last failure on the calling thread.  Arrays are
+ * plain pointers: a "field" is the reference's
+ *     complex(sp|dp), dimension(lx1, ly, lz)      lx1 = nx/2 + 1
+ * column-major, i.e. interleaved (re, im) pairs with x fastest.  Precision
+ * is fixed per plan: SCL_SP -> float pairs, SCL_DP -> double pairs.  Sizes
+ * are passed by value.  The reference's Fortran module set (fortran/) binds
+ * these through iso_c_binding under the reference's own procedure names.
+ *
+ * Grids: nx, ny, nz powers of two in [16, 1024] (FP64 plans: [16, 256]).
+ * The wavenumber arrays a reference routine takes (kx, ky, kz, k2) must be
+ * the wavenumber() ones (wavenumber.f90:89-129); the library checks the 1D
+ * arrays and builds k2 itself.
+ *
+ * Threading: a plan (and a solver built on it) is used from one host thread
+ * at a time; distinct plans are independent.  Work runs on the plan's HIP
+ * stream; host-array entry points return after their results are on the host.
+ */
+#ifndef SCL_H
+#define SCL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SCL_OK = 0,
+  SCL_EINVAL = -1,    /* bad argument (size, null pointer, k arrays) */
+  SCL_ENOMEM = -2,    /* device allocation failed */
+  SCL_EHIP = -3,      /* HIP runtime error */
+  SCL_ENOPLAN = -4,   /* no plan: call scl_fftwplan3d first */
+  SCL_ENONFINITE = -5,/* non-finite value found in the solution */
+  SCL_ENODEV = -6     /* no usable GPU */
+};
+
+enum { SCL_SP = 0, SCL_DP = 1 };
+
+typedef struct scl_plan3d scl_plan3d;
+typedef struct scl_dns scl_dns;
+
+const char *scl_last_error(void);
+int32_t scl_version(void);
+
+/* ---------------------------------------------------------------------
+ * Plan layer.  Replaces module mfftwplan3d (fftwplan.f90:1-31): the
+ * reference plans FFTW2 in-place r2c/c2r transforms into module globals
+ * R2C3D/C2R3D; here a plan holds the twiddle table, the stream and scratch.
+ * ------------------------------------------------------------------- */
+/* explicit handle */
+int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t prec, scl_plan3d **plan);
+int32_t scl_plan3d_destroy(scl_plan3d *plan);
+/* Use an existing HIP stream (hipStream_t passed as void*), NULL = own. */
+int32_t scl_plan3d_set_stream(scl_plan3d *plan, void *stream);
+/* fftwplan3d(ix,iy,iz) (fftwplan.f90:7-15) / fftwplan3de (:17-25): the
+ * process-global plan that the reference-named entry points use when passed
+ * plan = NULL.  scl_fftwplan3d makes an SP plan (constant_sp.f90). */
+int32_t scl_fftwplan3d(int32_t ix, int32_t iy, int32_t iz);
+int32_t scl_fftwplan3d_prec(int32_t ix, int32_t iy, int32_t iz, int32_t prec);
+/* destroyplan3d (fftwplan.f90:27-30) */
+int32_t scl_destroyplan3d(void);
+
+/* rfftwnd_f77_one_complex_to_real(C2R3D, a, ignore) / _real_to_complex
+ * (convec_dns.f90:22-27,45-47): unnormalised, in place, on a host array
+ * whose real view is (nx+2, ny, nz) (FFTW's padded in-place layout).
+ * c2r uses exp(+ik.x), r2c exp(-ik.x); c2r(r2c(a)) = nx*ny*nz*a. */
+int32_t scl_c2r3d(scl_plan3d *plan, void *a);
+int32_t scl_r2c3d(scl_plan3d *plan, void *a);
+
+/* ---------------------------------------------------------------------
+ * RHS blocks on host arrays (reference signatures; plan = NULL uses the
+ * global plan).
+ * ------------------------------------------------------------------- */
+/* wavenumber(kx,ky,kz,k2,lx1,ly,lz) (wavenumber.f90:89-129), host only.
+ * kx[lx1], ky[ly], kz[lz], k2[lx1*ly*lz] (k2 may be NULL); real type of prec. */
+int32_t scl_wavenumber3d(int32_t prec, void *kx, void *ky, void *kz, void *k2, int32_t lx1, int32_t ly, int32_t lz);
+
+/* convec_dns(vx,vy,vz,wx,wy,wz,lx1,ly,lz,nx,ny,nz) (convec_dns.f90:1-55):
+ * w <- FFT(u x w)/(nx ny nz), u = c2r(v), w = c2r(w). */
+int32_t scl_convec_dns(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, void *wx, void *wy,
+                       void *wz, int32_t lx1, int32_t ly, int32_t lz, int32_t nx, int32_t ny, int32_t nz);
+
+/* force(wx,wy,wz,vx,vy,vz,k2,lx1,ly,lz,fkmax,eps) (force.f90:3-31):
+ * w += eps/(2 E_f) v on k2 < fkmax^2.  k2 must be wavenumber()'s. */
+int32_t scl_force(scl_plan3d *plan, void *wx, void *wy, void *wz, const void *vx, const void *vy, const void *vz,
+                  const void *k2, int32_t lx1, int32_t ly, int32_t lz, double fkmax, double eps);
+
+/* force_heli(fx,fy,fz,vx,vy,vz,wx,wy,wz,k2,lx1,ly,lz,fkmax,eps,eta)
+ * (force_heli.f90:1-57): f = a v + b w on k2 < fkmax^2, 0 elsewhere. */
+int32_t scl_force_heli(scl_plan3d *plan, void *fx, void *fy, void *fz, const void *vx, const void *vy,
+                       const void *vz, const void *wx, const void *wy, const void *wz, const void *k2, int32_t lx1,
+                       int32_t ly, int32_t lz, double fkmax, double eps, double eta);
+
+/* symmetrize(c,lx1,ly,lz) (symmetrize.f90:32-64) */
+int32_t scl_symmetrize3d(scl_plan3d *plan, void *c, int32_t lx1, int32_t ly, int32_t lz);
+
+/* dealiasing(c,k2,lx1,ly,lz) (dealiasing.f90:2-37) */
+int32_t scl_dealiasing(scl_plan3d *plan, void *c, const void *k2, int32_t lx1, int32_t ly, int32_t lz);
+
+/* projection(vx,vy,vz,kx,ky,kz,lx1,ly,lz) (projection.f90:2-31) */
+int32_t scl_projection(scl_plan3d *plan, void *vx, void *vy, void *vz, const void *kx, const void *ky,
+                       const void *kz, int32_t lx1, int32_t ly, int32_t lz);
+
+/* ratesdudt (dns-deform.f90:259-288): the whole RHS N(v) = P S [u x w + f]
+ * into n{x,y,z}.  forced: 0 none, 1 force(), 2 force_heli(). */
+int32_t scl_ratesdudt(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, void *nx_, void *ny_,
+                      void *nz_, int32_t forced, double fkmax, double eps, double eta);
+
+/* initialize(vx,vy,vz,kx,ky,kz,k2,iseed,lx1,ly,lz,akp,u0,kcut)
+ * (initialize.f90:373-463 draw order, :328-332 complex projection), host
+ * only, in the real type of prec.  kcut <= 0 selects 2*lx/3. */
+int32_t scl_initialize3d(int32_t prec, void *vx, void *vy, void *vz, int32_t nx, int32_t ny, int32_t nz,
+                         int32_t iseed, double akp, double u0, double kcut);
+
+/* One Lawson IF-RK4 step + 2/3 truncation on host arrays, the inline step
+ * of dns-deform.f90:160-237 as a subroutine. */
+int32_t scl_rk4step(scl_plan3d *plan, void *vx, void *vy, void *vz, double nu, double dt, int32_t forced,
+                    double fkmax, double eps, double eta);
+
+/* ---------------------------------------------------------------------
+ * Device-resident solver (the dns-deform.f90 loop body, :113-241).
+ * ------------------------------------------------------------------- */
+typedef struct scl_dns_params {
+  double nu;           /* rnu */
+  double dt;
+  int32_t forced;      /* 0 none, 1 constant power (force.f90), 2 helical (force_heli.f90) */
+  double fkmax, eps, eta;
+  int32_t dealias_rhs; /* 1: also zero k2 >= kcut^2 in every RHS (default 0: once per step) */
+  int32_t use_graph;   /* 1: replay one captured step as a hipGraph */
+} scl_dns_params;
+
+typedef struct scl_diag {
+  double energy;       /* sum' |v|^2, weight 1/2 on kx=0 (dns-deform.f90:124-126) */
+  double dissipation;  /* 2 nu sum' k^2 |v|^2 */
+  double enstrophy;    /* sum' k^2 |v|^2 */
+  double helicity;     /* sum' 2 Re(w . conj v) (helispec.f90:82) */
+  double divmax;       /* max over the grid of |div u| (physical space) */
+  double divrms_spec;  /* sqrt(sum' |k.v|^2) */
+  int32_t nonfinite;   /* 1 if any NaN/Inf in v */
+} scl_diag;
+
+int32_t scl_dns_create(scl_plan3d *plan, const scl_dns_params *p, scl_dns **out);
+int32_t scl_dns_destroy(scl_dns *d);
+int32_t scl_dns_set_velocity(scl_dns *d, const void *vx, const void *vy, const void *vz);
+int32_t scl_dns_get_velocity(scl_dns *d, void *vx, void *vy, void *vz);
+int32_t scl_dns_initialize(scl_dns *d, int32_t iseed, double akp, double u0);
+/* nsteps IF-RK4 steps, asynchronous on the plan's stream */
+int32_t scl_dns_step(scl_dns *d, int32_t nsteps);
+int32_t scl_dns_sync(scl_dns *d);
+double scl_dns_time(const scl_dns *d);
+/* E(ll), ll = 1..nbins (nbins <= nx/2), FP64 (spec.f90:66-89) */
+int32_t scl_dns_spectrum(scl_dns *d, double *ek, int32_t nbins);
+/* H(ll) (helispec.f90:67-91) */
+int32_t scl_dns_helicity_spectrum(scl_dns *d, double *hk, int32_t nbins);
+int32_t scl_dns_diagnostics(scl_dns *d, scl_diag *out);
+
+/* Per-kernel timing of nsteps steps with HIP events on the plan's stream
+ * (non-graph launches).  kind_ms[k] / kind_count[k] for k < SCL_NKIND,
+ * total_ms = event time of the whole run. */
+enum {
+  SCL_K_FORCE = 0, SCL_K_ZINV = 1, SCL_K_YINV = 2, SCL_K_XPASS = 3, SCL_K_YFWD = 4, SCL_K_ZFWD = 5,
+  SCL_K_HERM = 6, SCL_NKIND = 7
+};
+int32_t scl_dns_step_timed(scl_dns *d, int32_t nsteps, double *kind_ms, int32_t *kind_count, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCL_H */

@@ -72,9 +72,9 @@ void oracle_gauss3_f64(int32_t seed, int64_t nmodes, double *g /* 2*3*nmodes */)
       double t2 = am * ran1_next(&s);
       if (t2 > rnmx) t2 = rnmx;
       t2 = 2.0 * pi * t2;
-      double r = sqrt(-2.0 * log(t1)) / sqrt(2.0);
-      g[2 * (3 * m + c) + 0] = r * cos(t2);
-      g[2 * (3 * m + c) + 1] = r * sin(t2);
+      double r = sqrt(-2.0 * log(t1));
+      g[2 * (3 * m + c) + 0] = r * cos(t2) / sqrt(2.0);
+      g[2 * (3 * m + c) + 1] = r * sin(t2) / sqrt(2.0);
     }
   }
 }
@@ -91,9 +91,9 @@ void oracle_gauss3_f32(int32_t seed, int64_t nmodes, float *g) {
       float t2 = am * (float)ran1_next(&s);
       if (t2 > rnmx) t2 = rnmx;
       t2 = 2.0f * pi * t2;
-      float r = sqrtf(-2.0f * logf(t1)) / sqrtf(2.0f);
-      g[2 * (3 * m + c) + 0] = r * cosf(t2);
-      g[2 * (3 * m + c) + 1] = r * sinf(t2);
+      float r = sqrtf(-2.0f * logf(t1));
+      g[2 * (3 * m + c) + 0] = r * cosf(t2) / sqrtf(2.0f);
+      g[2 * (3 * m + c) + 1] = r * sinf(t2) / sqrtf(2.0f);
     }
   }
 }

@@ -1,0 +1,315 @@
+"""speccodelib_amd: MI355X pseudo-spectral RHS + IF-RK4 path behind the
+reference's routine names.
+
+Host-side mirror of the reference interface over the C ABI (include/scl.h).
+Arrays follow the reference layout: a Fortran ``complex(sp), dimension(lx1,
+ly, lz)`` is a C-ordered numpy array of shape ``(lz, ly, lx1)``
+(``complex64`` for SP plans, ``complex128`` for DP plans).
+
+Nothing here computes on the CPU: every numerical routine runs through
+``libscl.so`` on the GPU, and importing a routine without the built library
+raises ``RuntimeError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libscl.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "scl.h")
+
+SP, DP = 0, 1
+_STATUS = {0: "SCL_OK", -1: "SCL_EINVAL", -2: "SCL_ENOMEM", -3: "SCL_EHIP", -4: "SCL_ENOPLAN", -5: "SCL_ENONFINITE", -6: "SCL_ENODEV"}
+KIND_NAMES = ["force_coef", "zinv_vort", "y_inverse", "x_pass", "y_forward", "zfwd_rk", "herm_fix"]
+
+_lib = None
+
+
+class SclError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ScDnsParams(ctypes.Structure):
+    _fields_ = [
+        ("nu", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("forced", ctypes.c_int32),
+        ("fkmax", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("eta", ctypes.c_double),
+        ("dealias_rhs", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
+class ScDiag(ctypes.Structure):
+    _fields_ = [
+        ("energy", ctypes.c_double),
+        ("dissipation", ctypes.c_double),
+        ("enstrophy", ctypes.c_double),
+        ("helicity", ctypes.c_double),
+        ("divmax", ctypes.c_double),
+        ("divrms_spec", ctypes.c_double),
+        ("nonfinite", ctypes.c_int32),
+    ]
+
+
+def header_functions() -> list[str]:
+    """Names of the functions include/scl.h declares."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(scl_\w+)\s*\(", txt, flags=re.M)))
+
+
+def lib():
+    """The loaded libscl.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        P, I32, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+        sig = {
+            "scl_last_error": ([], ctypes.c_char_p),
+            "scl_version": ([], I32),
+            "scl_plan3d_create": ([I32, I32, I32, I32, ctypes.POINTER(P)], I32),
+            "scl_plan3d_destroy": ([P], I32),
+            "scl_plan3d_set_stream": ([P, P], I32),
+            "scl_fftwplan3d": ([I32, I32, I32], I32),
+            "scl_fftwplan3d_prec": ([I32, I32, I32, I32], I32),
+            "scl_destroyplan3d": ([], I32),
+            "scl_c2r3d": ([P, P], I32),
+            "scl_r2c3d": ([P, P], I32),
+            "scl_wavenumber3d": ([I32, P, P, P, P, I32, I32, I32], I32),
+            "scl_convec_dns": ([P] + [P] * 6 + [I32] * 6, I32),
+            "scl_force": ([P] + [P] * 7 + [I32] * 3 + [D, D], I32),
+            "scl_force_heli": ([P] + [P] * 10 + [I32] * 3 + [D, D, D], I32),
+            "scl_symmetrize3d": ([P, P, I32, I32, I32], I32),
+            "scl_dealiasing": ([P, P, P, I32, I32, I32], I32),
+            "scl_projection": ([P] + [P] * 6 + [I32] * 3, I32),
+            "scl_ratesdudt": ([P] + [P] * 6 + [I32, D, D, D], I32),
+            "scl_initialize3d": ([I32, P, P, P, I32, I32, I32, I32, D, D, D], I32),
+            "scl_rk4step": ([P, P, P, P, D, D, I32, D, D, D], I32),
+            "scl_dns_create": ([P, ctypes.POINTER(ScDnsParams), ctypes.POINTER(P)], I32),
+            "scl_dns_destroy": ([P], I32),
+            "scl_dns_set_velocity": ([P, P, P, P], I32),
+            "scl_dns_get_velocity": ([P, P, P, P], I32),
+            "scl_dns_initialize": ([P, I32, D, D], I32),
+            "scl_dns_step": ([P, I32], I32),
+            "scl_dns_sync": ([P], I32),
+            "scl_dns_time": ([P], D),
+            "scl_dns_spectrum": ([P, P, I32], I32),
+            "scl_dns_helicity_spectrum": ([P, P, I32], I32),
+            "scl_dns_diagnostics": ([P, ctypes.POINTER(ScDiag)], I32),
+            "scl_dns_step_timed": ([P, I32, P, P, P], I32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise SclError(status, lib().scl_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _cdtype(prec):
+    return np.complex64 if prec == SP else np.complex128
+
+
+def _rdtype(prec):
+    return np.float32 if prec == SP else np.float64
+
+
+def _field(a, prec, shape):
+    if a.dtype != _cdtype(prec) or a.shape != shape or not a.flags.c_contiguous:
+        raise ValueError(f"expected C-contiguous {np.dtype(_cdtype(prec))} array of shape {shape}, got {a.dtype} {a.shape}")
+    return a
+
+
+class Plan3D:
+    """A plan (fftwplan.f90:7-15): grid, precision, stream, twiddles."""
+
+    def __init__(self, nx, ny=None, nz=None, prec=SP):
+        ny = ny or nx
+        nz = nz or nx
+        h = ctypes.c_void_p()
+        _check(lib().scl_plan3d_create(nx, ny, nz, prec, ctypes.byref(h)))
+        self.h, self.nx, self.ny, self.nz, self.prec = h, nx, ny, nz, prec
+        self.shape = (nz, ny, nx // 2 + 1)
+
+    def close(self):
+        if self.h:
+            lib().scl_plan3d_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def zeros(self):
+        return np.zeros(self.shape, _cdtype(self.prec))
+
+    # -- FFTW-equivalent in-place transforms (real view (nx+2, ny, nz)) --
+    def c2r(self, a):
+        """rfftwnd_f77_one_complex_to_real: in place; returns the real view
+        (nz, ny, nx) of the result."""
+        _field(a, self.prec, self.shape)
+        _check(lib().scl_c2r3d(self.h, _ptr(a)))
+        return a.view(_rdtype(self.prec)).reshape(self.nz, self.ny, self.nx + 2)[:, :, : self.nx]
+
+    def r2c(self, a):
+        """rfftwnd_f77_one_real_to_complex: in place on the complex view."""
+        _field(a, self.prec, self.shape)
+        _check(lib().scl_r2c3d(self.h, _ptr(a)))
+        return a
+
+    def packed_real(self, r):
+        """A complex array whose real view holds the real field r (nz, ny, nx)
+        with FFTW's two padding reals per x-line."""
+        a = self.zeros()
+        a.view(_rdtype(self.prec)).reshape(self.nz, self.ny, self.nx + 2)[:, :, : self.nx] = r
+        return a
+
+    # -- RHS blocks (reference signatures, arrays updated in place) --
+    def convec_dns(self, vx, vy, vz, wx, wy, wz):
+        for a in (vx, vy, vz, wx, wy, wz):
+            _field(a, self.prec, self.shape)
+        lx1, ly, lz = self.nx // 2 + 1, self.ny, self.nz
+        _check(lib().scl_convec_dns(self.h, *map(_ptr, (vx, vy, vz, wx, wy, wz)), lx1, ly, lz, self.nx, self.ny, self.nz))
+
+    def force(self, wx, wy, wz, vx, vy, vz, k2, fkmax, eps):
+        for a in (vx, vy, vz, wx, wy, wz):
+            _field(a, self.prec, self.shape)
+        _check(lib().scl_force(self.h, *map(_ptr, (wx, wy, wz, vx, vy, vz, k2)), *self._dims(), fkmax, eps))
+
+    def force_heli(self, vx, vy, vz, wx, wy, wz, k2, fkmax, eps, eta):
+        out = [self.zeros() for _ in range(3)]
+        _check(lib().scl_force_heli(self.h, *map(_ptr, (*out, vx, vy, vz, wx, wy, wz, k2)), *self._dims(), fkmax, eps, eta))
+        return tuple(out)
+
+    def symmetrize(self, c):
+        _field(c, self.prec, self.shape)
+        _check(lib().scl_symmetrize3d(self.h, _ptr(c), *self._dims()))
+
+    def dealiasing(self, c, k2):
+        _field(c, self.prec, self.shape)
+        _check(lib().scl_dealiasing(self.h, _ptr(c), _ptr(k2), *self._dims()))
+
+    def projection(self, vx, vy, vz, kx, ky, kz):
+        for a in (vx, vy, vz):
+            _field(a, self.prec, self.shape)
+        _check(lib().scl_projection(self.h, *map(_ptr, (vx, vy, vz, kx, ky, kz)), *self._dims()))
+
+    def ratesdudt(self, vx, vy, vz, forced=1, fkmax=2.0, eps=0.1, eta=0.0):
+        out = [self.zeros() for _ in range(3)]
+        _check(lib().scl_ratesdudt(self.h, *map(_ptr, (vx, vy, vz, *out)), forced, fkmax, eps, eta))
+        return tuple(out)
+
+    def rk4step(self, vx, vy, vz, nu, dt, forced=1, fkmax=2.0, eps=0.1, eta=0.0):
+        _check(lib().scl_rk4step(self.h, *map(_ptr, (vx, vy, vz)), nu, dt, forced, fkmax, eps, eta))
+
+    def wavenumber(self):
+        return wavenumber(self.nx, self.ny, self.nz, self.prec)
+
+    def _dims(self):
+        return self.nx // 2 + 1, self.ny, self.nz
+
+
+def wavenumber(nx, ny=None, nz=None, prec=SP):
+    """wavenumber(kx,ky,kz,k2,lx1,ly,lz) (wavenumber.f90:89-129)."""
+    ny = ny or nx
+    nz = nz or nx
+    rd = _rdtype(prec)
+    kx = np.zeros(nx // 2 + 1, rd)
+    ky = np.zeros(ny, rd)
+    kz = np.zeros(nz, rd)
+    k2 = np.zeros((nz, ny, nx // 2 + 1), rd)
+    _check(lib().scl_wavenumber3d(prec, _ptr(kx), _ptr(ky), _ptr(kz), _ptr(k2), nx // 2 + 1, ny, nz))
+    return kx, ky, kz, k2
+
+
+def initialize(nx, ny=None, nz=None, iseed=1, akp=4.0, u0=1.0, prec=SP, kcut=-1.0):
+    """initialize(vx,vy,vz,...,iseed,...,akp,u0,kcut) (initialize.f90:373-463,
+    with the complex projection of :328-332)."""
+    ny = ny or nx
+    nz = nz or nx
+    out = [np.zeros((nz, ny, nx // 2 + 1), _cdtype(prec)) for _ in range(3)]
+    _check(lib().scl_initialize3d(prec, *map(_ptr, out), nx, ny, nz, iseed, akp, u0, kcut))
+    return tuple(out)
+
+
+class DNS:
+    """Device-resident IF-RK4 solver: the dns-deform.f90 loop body."""
+
+    def __init__(self, plan: Plan3D, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, eta=0.0, dealias_rhs=False, use_graph=True):
+        self.plan = plan
+        self.params = ScDnsParams(nu, dt, int(forced), fkmax, eps, eta, int(dealias_rhs), int(use_graph))
+        h = ctypes.c_void_p()
+        _check(lib().scl_dns_create(plan.h, ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().scl_dns_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_velocity(self, vx, vy, vz):
+        for a in (vx, vy, vz):
+            _field(a, self.plan.prec, self.plan.shape)
+        _check(lib().scl_dns_set_velocity(self.h, _ptr(vx), _ptr(vy), _ptr(vz)))
+
+    def get_velocity(self):
+        out = [self.plan.zeros() for _ in range(3)]
+        _check(lib().scl_dns_get_velocity(self.h, *map(_ptr, out)))
+        return tuple(out)
+
+    def initialize(self, iseed=1, akp=4.0, u0=1.0):
+        _check(lib().scl_dns_initialize(self.h, iseed, akp, u0))
+
+    def step(self, n=1):
+        _check(lib().scl_dns_step(self.h, n))
+
+    def sync(self):
+        _check(lib().scl_dns_sync(self.h))
+
+    @property
+    def time(self):
+        return lib().scl_dns_time(self.h)
+
+    def spectrum(self):
+        ek = np.zeros(self.plan.nx // 2)
+        _check(lib().scl_dns_spectrum(self.h, _ptr(ek), len(ek)))
+        return ek
+
+    def helicity_spectrum(self):
+        hk = np.zeros(self.plan.nx // 2)
+        _check(lib().scl_dns_helicity_spectrum(self.h, _ptr(hk), len(hk)))
+        return hk
+
+    def diagnostics(self) -> dict:
+        d = ScDiag()
+        _check(lib().scl_dns_diagnostics(self.h, ctypes.byref(d)))
+        return {k: getattr(d, k) for k, _ in ScDiag._fields_}
+
+    def step_timed(self, n):
+        """Runs n steps with HIP events around every kernel launch; returns
+        (total_ms, {kind: (ms, launches)})."""
+        ms = np.zeros(len(KIND_NAMES))
+        cnt = np.zeros(len(KIND_NAMES), np.int32)
+        tot = ctypes.c_double()
+        _check(lib().scl_dns_step_timed(self.h, n, _ptr(ms), _ptr(cnt), ctypes.byref(tot)))
+        return tot.value, {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KIND_NAMES)}

@@ -1,0 +1,881 @@
+// C ABI: plans, host-array entry points under the reference's routine names,
+// and the device-resident IF-RK4 solver.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <vector>
+
+#include "diag.hpp"
+#include "ic.hpp"
+#include "ops.hpp"
+
+namespace scl {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char* last_error() { return g_err; }
+
+int32_t check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+    return SCL_EHIP;
+  }
+  return SCL_OK;
+}
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+template <class F>
+static int32_t by_prec(int prec, F&& f) {
+  if (prec == SCL_SP) return f(float{});
+  return f(double{});
+}
+
+// exp(-2 pi i j / L) for j < L, evaluated in FP64 with octant symmetry so
+// that the quarter- and eighth-turn values are exact images of each other.
+static void twiddles(int L, std::vector<double>& re, std::vector<double>& im) {
+  re.resize(L);
+  im.resize(L);
+  const double twopi = 6.283185307179586476925286766559;
+  const int Q = L / 4;
+  for (int j = 0; j < L; ++j) {
+    const int q = j / Q, r = j % Q;
+    double c, s;
+    if (2 * r <= Q) {
+      c = std::cos(twopi * r / L);
+      s = std::sin(twopi * r / L);
+    } else {
+      c = std::sin(twopi * (Q - r) / L);
+      s = std::cos(twopi * (Q - r) / L);
+    }
+    if (r == 0) {
+      c = 1.0;
+      s = 0.0;
+    }
+    double cr, sr;  // rotate (c, s) by q quarter turns
+    switch (q) {
+      case 0: cr = c; sr = s; break;
+      case 1: cr = -s; sr = c; break;
+      case 2: cr = -c; sr = -s; break;
+      default: cr = s; sr = -c; break;
+    }
+    re[j] = cr;
+    im[j] = -sr;  // exp(-i theta)
+  }
+}
+
+int32_t plan_scratch(scl_plan3d* p, size_t nfields, void** out) {
+  if (!p->scratch || p->scratch_fields < nfields) {
+    if (p->scratch) {
+      hipStreamSynchronize(p->stream);
+      hipFree(p->scratch);
+      p->scratch = nullptr;
+      p->scratch_fields = 0;
+    }
+    const size_t bytes = nfields * (size_t)p->fstride() * p->esize() + p->ref_bytes();
+    if (hipMalloc(&p->scratch, bytes) != hipSuccess) {
+      hipGetLastError();
+      set_error("device allocation of %zu bytes failed", bytes);
+      return SCL_ENOMEM;
+    }
+    p->scratch_fields = nfields;
+  }
+  *out = p->scratch;
+  return SCL_OK;
+}
+
+int32_t plan_diag(scl_plan3d* p, size_t bytes, void** out) {
+  if (p->diag_bytes < bytes) {
+    if (p->diag) {
+      hipStreamSynchronize(p->stream);
+      hipFree(p->diag);
+      p->diag = nullptr;
+      p->diag_bytes = 0;
+    }
+    if (hipMalloc(&p->diag, bytes) != hipSuccess) {
+      hipGetLastError();
+      set_error("device allocation of %zu bytes failed", bytes);
+      return SCL_ENOMEM;
+    }
+    p->diag_bytes = bytes;
+  }
+  *out = p->diag;
+  return SCL_OK;
+}
+
+// staging buffer of the plan's scratch (after nfields work fields)
+template <class R>
+static cpx<R>* staging(scl_plan3d* p) {
+  return reinterpret_cast<cpx<R>*>(p->scratch) + p->scratch_fields * p->fstride();
+}
+
+template <class R>
+static int32_t upload(scl_plan3d* p, const void* host, cpx<R>* dst) {
+  if (!host) {
+    set_error("null host array");
+    return SCL_EINVAL;
+  }
+  cpx<R>* st = staging<R>(p);
+  SCL_HIP(hipMemcpyAsync(st, host, p->ref_bytes(), hipMemcpyHostToDevice, p->stream));
+  return Ops<R>::ref_to_split(*p, st, dst);
+}
+
+template <class R>
+static int32_t download(scl_plan3d* p, const cpx<R>* src, void* host) {
+  if (!host) {
+    set_error("null host array");
+    return SCL_EINVAL;
+  }
+  cpx<R>* st = staging<R>(p);
+  SCL_CHECK(Ops<R>::split_to_ref(*p, src, st));
+  SCL_HIP(hipMemcpyAsync(host, st, p->ref_bytes(), hipMemcpyDeviceToHost, p->stream));
+  SCL_HIP(hipStreamSynchronize(p->stream));
+  return SCL_OK;
+}
+
+static std::mutex g_plan_mu;
+static scl_plan3d* g_plan = nullptr;
+
+static scl_plan3d* resolve(scl_plan3d* p) {
+  if (p) return p;
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  return g_plan;
+}
+
+#define SCL_PLAN(p)                                                 \
+  do {                                                              \
+    p = resolve(p);                                                 \
+    if (!p) {                                                       \
+      set_error("no plan: call scl_fftwplan3d (fftwplan3d) first"); \
+      return SCL_ENOPLAN;                                           \
+    }                                                               \
+  } while (0)
+
+static int32_t check_dims(const scl_plan3d* p, int lx1, int ly, int lz) {
+  if (lx1 != p->g.mx + 1 || ly != p->g.ny || lz != p->g.nz) {
+    set_error("array shape (%d,%d,%d) does not match the plan (%d,%d,%d)", lx1, ly, lz, p->g.mx + 1, p->g.ny,
+              p->g.nz);
+    return SCL_EINVAL;
+  }
+  return SCL_OK;
+}
+
+// The wavenumber arrays passed to a reference routine must be wavenumber()'s.
+template <class R>
+static int32_t check_k1d(const scl_plan3d* p, const void* kx, const void* ky, const void* kz) {
+  const R* a = (const R*)kx;
+  const R* b = (const R*)ky;
+  const R* c = (const R*)kz;
+  if (!a || !b || !c) {
+    set_error("null wavenumber array");
+    return SCL_EINVAL;
+  }
+  for (int i = 0; i <= p->g.mx; ++i)
+    if (a[i] != (R)i) return set_error("kx(%d) is not wavenumber()'s", i + 1), SCL_EINVAL;
+  for (int i = 0; i < p->g.ny; ++i)
+    if (b[i] != (R)kwave(i, p->g.ny)) return set_error("ky(%d) is not wavenumber()'s", i + 1), SCL_EINVAL;
+  for (int i = 0; i < p->g.nz; ++i)
+    if (c[i] != (R)kwave(i, p->g.nz)) return set_error("kz(%d) is not wavenumber()'s", i + 1), SCL_EINVAL;
+  return SCL_OK;
+}
+
+template <class R>
+static int32_t check_k2(const scl_plan3d* p, const void* k2v) {
+  const R* k2 = (const R*)k2v;
+  if (!k2) return set_error("null k2"), SCL_EINVAL;
+  const Geom& g = p->g;
+  // spot checks at the corners and a few interior points
+  const int pts[6][3] = {{0, 0, 0}, {1, 0, 0}, {g.mx, g.ny - 1, g.nz - 1}, {3 % (g.mx + 1), 2, 1},
+                         {g.mx / 2, g.ny / 2, g.nz / 3}, {1, g.ny - 1, 1}};
+  for (auto& q : pts) {
+    const int x = q[0], y = q[1], z = q[2];
+    const int ik2 = x * x + kwave(y, g.ny) * kwave(y, g.ny) + kwave(z, g.nz) * kwave(z, g.nz);
+    const R want = ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2;
+    if (k2[((int64_t)z * g.ny + y) * (g.mx + 1) + x] != want)
+      return set_error("k2(%d,%d,%d) is not wavenumber()'s", x + 1, y + 1, z + 1), SCL_EINVAL;
+  }
+  return SCL_OK;
+}
+
+}  // namespace scl
+
+using namespace scl;
+
+// ---------------------------------------------------------------------------
+extern "C" const char* scl_last_error(void) { return scl::last_error(); }
+extern "C" int32_t scl_version(void) { return 100; }
+
+extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t prec, scl_plan3d** out) {
+  if (!out) return set_error("null plan pointer"), SCL_EINVAL;
+  *out = nullptr;
+  const int hi = prec == SCL_SP ? 1024 : 256;
+  if (prec != SCL_SP && prec != SCL_DP) return set_error("bad precision %d", prec), SCL_EINVAL;
+  if (!pow2_in(nx, 16, hi) || !pow2_in(ny, 16, hi) || !pow2_in(nz, 16, hi))
+    return set_error("grid (%d,%d,%d): each size must be a power of two in [16,%d]", nx, ny, nz, hi), SCL_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    hipGetLastError();
+    return set_error("no HIP device"), SCL_ENODEV;
+  }
+  auto* p = new scl_plan3d();
+  p->prec = prec;
+  p->g = make_geom(nx, ny, nz);
+  hipGetDevice(&p->device);
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete p;
+    return set_error("stream creation failed"), SCL_EHIP;
+  }
+  p->own_stream = true;
+  p->L = std::max(nx, std::max(ny, nz));
+  std::vector<double> re, im;
+  twiddles(p->L, re, im);
+  int32_t st = by_prec(prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    std::vector<cpx<R>> h(p->L);
+    for (int j = 0; j < p->L; ++j) h[j] = mk<R>((R)re[j], (R)im[j]);
+    SCL_HIP(hipMalloc(&p->tw, p->L * sizeof(cpx<R>)));
+    SCL_HIP(hipMemcpy(p->tw, h.data(), p->L * sizeof(cpx<R>), hipMemcpyHostToDevice));
+    SCL_HIP(hipMalloc((void**)&p->coef, 8 * sizeof(double)));
+    SCL_HIP(hipMemset(p->coef, 0, 8 * sizeof(double)));
+    return SCL_OK;
+  });
+  if (st != SCL_OK) {
+    scl_plan3d_destroy(p);
+    return st;
+  }
+  *out = p;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_plan3d_destroy(scl_plan3d* p) {
+  if (!p) return SCL_OK;
+  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->tw) hipFree(p->tw);
+  if (p->coef) hipFree(p->coef);
+  if (p->scratch) hipFree(p->scratch);
+  if (p->diag) hipFree(p->diag);
+  if (p->stream && p->own_stream) hipStreamDestroy(p->stream);
+  delete p;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_plan3d_set_stream(scl_plan3d* p, void* stream) {
+  if (!p) return set_error("null plan"), SCL_EINVAL;
+  hipStreamSynchronize(p->stream);
+  if (p->own_stream) hipStreamDestroy(p->stream);
+  if (stream) {
+    p->stream = (hipStream_t)stream;
+    p->own_stream = false;
+  } else {
+    SCL_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->own_stream = true;
+  }
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_fftwplan3d_prec(int32_t ix, int32_t iy, int32_t iz, int32_t prec) {
+  scl_plan3d* p = nullptr;
+  SCL_CHECK(scl_plan3d_create(ix, iy, iz, prec, &p));
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  if (g_plan) scl_plan3d_destroy(g_plan);
+  g_plan = p;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_fftwplan3d(int32_t ix, int32_t iy, int32_t iz) { return scl_fftwplan3d_prec(ix, iy, iz, SCL_SP); }
+
+extern "C" int32_t scl_destroyplan3d(void) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  scl_plan3d_destroy(g_plan);
+  g_plan = nullptr;
+  return SCL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FFTW-equivalent transforms on host arrays (real view (nx+2, ny, nz))
+extern "C" int32_t scl_c2r3d(scl_plan3d* p, void* a) {
+  SCL_PLAN(p);
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 1, &s));
+    cpx<R>* f = (cpx<R>*)s;
+    SCL_CHECK(upload<R>(p, a, f));
+    SCL_CHECK(Ops<R>::c2r3d(*p, f, p->fstride(), 1));
+    return download<R>(p, f, a);
+  });
+}
+
+extern "C" int32_t scl_r2c3d(scl_plan3d* p, void* a) {
+  SCL_PLAN(p);
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 1, &s));
+    cpx<R>* f = (cpx<R>*)s;
+    SCL_CHECK(upload<R>(p, a, f));
+    SCL_CHECK(Ops<R>::r2c3d(*p, f, p->fstride(), 1));
+    return download<R>(p, f, a);
+  });
+}
+
+extern "C" int32_t scl_wavenumber3d(int32_t prec, void* kx, void* ky, void* kz, void* k2, int32_t lx1, int32_t ly,
+                                   int32_t lz) {
+  if (!kx || !ky || !kz || lx1 < 2 || ly < 2 || lz < 2) return set_error("bad wavenumber arguments"), SCL_EINVAL;
+  return by_prec(prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    R* a = (R*)kx;
+    R* b = (R*)ky;
+    R* c = (R*)kz;
+    for (int i = 0; i < lx1; ++i) a[i] = (R)i;
+    for (int i = 0; i < ly; ++i) b[i] = (R)kwave(i, ly);
+    for (int i = 0; i < lz; ++i) c[i] = (R)kwave(i, lz);
+    if (k2) {
+      R* d = (R*)k2;
+      for (int z = 0; z < lz; ++z)
+        for (int y = 0; y < ly; ++y)
+          for (int x = 0; x < lx1; ++x) d[((int64_t)z * ly + y) * lx1 + x] = a[x] * a[x] + b[y] * b[y] + c[z] * c[z];
+      d[0] = std::numeric_limits<R>::min();
+    }
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_convec_dns(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* wx, void* wy,
+                                  void* wz, int32_t lx1, int32_t ly, int32_t lz, int32_t nx, int32_t ny, int32_t nz) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  if (nx != p->g.nx || ny != p->g.ny || nz != p->g.nz) return set_error("nx,ny,nz do not match the plan"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 6, &s));
+    cpx<R>* W = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    const void* in[6] = {vx, vy, vz, wx, wy, wz};
+    for (int f = 0; f < 6; ++f) SCL_CHECK(upload<R>(p, in[f], W + f * fs));
+    SCL_CHECK(Ops<R>::colfft(*p, W, fs, 6, 2, +1));
+    SCL_CHECK(Ops<R>::colfft(*p, W, fs, 6, 1, +1));
+    SCL_CHECK(Ops<R>::row_convec(*p, W, fs));
+    SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
+    SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 2, -1));
+    void* out[3] = {wx, wy, wz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(download<R>(p, W + f * fs, out[f]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_symmetrize3d(scl_plan3d* p, void* c, int32_t lx1, int32_t ly, int32_t lz) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 1, &s));
+    cpx<R>* f = (cpx<R>*)s;
+    SCL_CHECK(upload<R>(p, c, f));
+    SCL_CHECK(Ops<R>::sym_zero(*p, f, (R)0));
+    FieldPtrs<R> fp{};
+    fp.p[0] = f;
+    fp.n = 1;
+    SCL_CHECK(Ops<R>::herm_fix(*p, fp));
+    return download<R>(p, f, c);
+  });
+}
+
+extern "C" int32_t scl_dealiasing(scl_plan3d* p, void* c, const void* k2, int32_t lx1, int32_t ly, int32_t lz) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k2<R>(p, k2));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 1, &s));
+    cpx<R>* f = (cpx<R>*)s;
+    SCL_CHECK(upload<R>(p, c, f));
+    SCL_CHECK(Ops<R>::sym_zero(*p, f, (R)0));
+    FieldPtrs<R> fp{};
+    fp.p[0] = f;
+    fp.n = 1;
+    SCL_CHECK(Ops<R>::herm_fix(*p, fp));
+    // dealiasing.f90:35: k2 >= real(ly*ly)/9
+    const R thr = (R)(ly * ly) / (R)9;
+    SCL_CHECK(Ops<R>::dealias_mask(*p, f, thr));
+    return download<R>(p, f, c);
+  });
+}
+
+extern "C" int32_t scl_projection(scl_plan3d* p, void* vx, void* vy, void* vz, const void* kx, const void* ky,
+                                  const void* kz, int32_t lx1, int32_t ly, int32_t lz) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k1d<R>(p, kx, ky, kz));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 3, &s));
+    cpx<R>* V = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    void* a[3] = {vx, vy, vz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, a[f], V + f * fs));
+    SCL_CHECK(Ops<R>::projection(*p, V, fs));
+    for (int f = 0; f < 3; ++f) SCL_CHECK(download<R>(p, V + f * fs, a[f]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_force(scl_plan3d* p, void* wx, void* wy, void* wz, const void* vx, const void* vy,
+                             const void* vz, const void* k2, int32_t lx1, int32_t ly, int32_t lz, double fkmax,
+                             double eps) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k2<R>(p, k2));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 6, &s));
+    cpx<R>* V = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    cpx<R>* W = V + 3 * fs;
+    const void* vi[3] = {vx, vy, vz};
+    void* wi[3] = {wx, wy, wz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, vi[f], V + f * fs));
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, wi[f], W + f * fs));
+    SCL_CHECK(Ops<R>::force_coef(*p, V, fs, fkmax, eps, 0.0, 1, p->coef));
+    const R fk = (R)fkmax;
+    SCL_CHECK(Ops<R>::force_apply(*p, W, V, fs, p->coef, fk * fk, 1));
+    for (int f = 0; f < 3; ++f) SCL_CHECK(download<R>(p, W + f * fs, wi[f]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_force_heli(scl_plan3d* p, void* fx, void* fy, void* fz, const void* vx, const void* vy,
+                                  const void* vz, const void* wx, const void* wy, const void* wz, const void* k2,
+                                  int32_t lx1, int32_t ly, int32_t lz, double fkmax, double eps, double eta) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k2<R>(p, k2));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 6, &s));
+    cpx<R>* V = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    cpx<R>* W = V + 3 * fs;
+    const void* vi[3] = {vx, vy, vz};
+    const void* wi[3] = {wx, wy, wz};
+    void* fo[3] = {fx, fy, fz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, vi[f], V + f * fs));
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, wi[f], W + f * fs));
+    SCL_CHECK(Ops<R>::force_coef(*p, V, fs, fkmax, eps, eta, 2, p->coef));
+    const R fk = (R)fkmax;
+    SCL_CHECK(Ops<R>::force_apply(*p, W, V, fs, p->coef, fk * fk, 2));
+    for (int f = 0; f < 3; ++f) SCL_CHECK(download<R>(p, W + f * fs, fo[f]));
+    return SCL_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// One RHS / one IF-RK4 step on device fields
+template <class R>
+static RhsParams<R> rhs_params(const scl_plan3d* p, double nu, double dt, int forced, double fkmax, int dealias) {
+  RhsParams<R> rp;
+  rp.dt = (R)dt;
+  rp.nu = (R)nu;
+  const R fk = (R)fkmax;
+  rp.fk2 = fk * fk;
+  // dns-deform.f90:75-76: kcut = 2*lx/3 (default real), kcut2 = kcut*kcut
+  const float kcut = 2.0f * (float)p->g.mx / 3.0f;
+  rp.kc2 = (R)kcut * (R)kcut;
+  rp.forced = forced;
+  rp.dealias = dealias;
+  return rp;
+}
+
+struct DevFields {
+  void* base;
+  int64_t fs;
+  template <class R>
+  cpx<R>* f(int i) const {
+    return reinterpret_cast<cpx<R>*>(base) + i * fs;
+  }
+};
+
+// Event hooks for per-kernel timing (nullptr: none).
+struct Timer {
+  std::vector<hipEvent_t>* ev = nullptr;
+  std::vector<int>* kinds = nullptr;
+  hipStream_t s = nullptr;
+  void mark(int kind) {
+    if (!ev) return;
+    hipEvent_t e;
+    hipEventCreate(&e);
+    hipEventRecord(e, s);
+    ev->push_back(e);
+    kinds->push_back(kind);
+  }
+};
+
+// RHS of stage `mode` (0: N(st) into W[0..2]) with the fields of a solver:
+// V = f(0..2), DU = f(3..5), S = f(6..8), W = f(9..14).
+template <class R>
+static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsParams<R>& rp, double eps, double eta,
+                        double fkmax, const cpx<R>* st, Timer* tm) {
+  const int64_t fs = d.fs;
+  cpx<R>* V = d.f<R>(0);
+  cpx<R>* DU = d.f<R>(3);
+  cpx<R>* S = d.f<R>(6);
+  cpx<R>* W = d.f<R>(9);
+  Timer none;
+  if (!tm) tm = &none;
+  if (rp.forced) {
+    tm->mark(SCL_K_FORCE);
+    SCL_CHECK(Ops<R>::force_coef(*p, st, fs, fkmax, eps, eta, rp.forced, p->coef));
+  }
+  tm->mark(SCL_K_ZINV);
+  SCL_CHECK(Ops<R>::zinv_vort(*p, st, fs, W, fs));
+  tm->mark(SCL_K_YINV);
+  SCL_CHECK(Ops<R>::colfft(*p, W, fs, 6, 1, +1));
+  tm->mark(SCL_K_XPASS);
+  SCL_CHECK(Ops<R>::row_convec(*p, W, fs));
+  tm->mark(SCL_K_YFWD);
+  SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
+  tm->mark(SCL_K_ZFWD);
+  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, p->coef, rp));
+  tm->mark(SCL_K_HERM);
+  FieldPtrs<R> fp{};
+  if (mode == 0) {
+    for (int q = 0; q < 3; ++q) fp.p[q] = W + q * fs;
+    fp.n = 3;
+  } else if (mode < 4) {
+    for (int q = 0; q < 3; ++q) {
+      fp.p[q] = DU + q * fs;
+      fp.p[3 + q] = S + q * fs;
+    }
+    fp.n = 6;
+  } else {
+    for (int q = 0; q < 3; ++q) fp.p[q] = V + q * fs;
+    fp.n = 3;
+  }
+  SCL_CHECK(Ops<R>::herm_fix(*p, fp));
+  tm->mark(-1);
+  return SCL_OK;
+}
+
+template <class R>
+static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm) {
+  const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  for (int stage = 1; stage <= 4; ++stage) {
+    const cpx<R>* st = stage == 1 ? d.f<R>(0) : d.f<R>(6);
+    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm));
+  }
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ratesdudt(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* nx_, void* ny_,
+                                 void* nz_, int32_t forced, double fkmax, double eps, double eta) {
+  SCL_PLAN(p);
+  if (forced < 0 || forced > 2) return set_error("forced must be 0, 1 or 2"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 15, &s));
+    DevFields d{s, p->fstride()};
+    const void* vi[3] = {vx, vy, vz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, vi[f], d.f<R>(6 + f)));
+    const RhsParams<R> rp = rhs_params<R>(p, 0.0, 0.0, forced, fkmax, 0);
+    SCL_CHECK(rk_stage<R>(p, d, 0, rp, eps, eta, fkmax, d.f<R>(6), nullptr));
+    void* out[3] = {nx_, ny_, nz_};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(download<R>(p, d.f<R>(9 + f), out[f]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_rk4step(scl_plan3d* p, void* vx, void* vy, void* vz, double nu, double dt, int32_t forced,
+                               double fkmax, double eps, double eta) {
+  SCL_PLAN(p);
+  if (forced < 0 || forced > 2) return set_error("forced must be 0, 1 or 2"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 15, &s));
+    DevFields d{s, p->fstride()};
+    void* vi[3] = {vx, vy, vz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, vi[f], d.f<R>(f)));
+    scl_dns_params prm{};
+    prm.nu = nu;
+    prm.dt = dt;
+    prm.forced = forced;
+    prm.fkmax = fkmax;
+    prm.eps = eps;
+    prm.eta = eta;
+    SCL_CHECK(rk4_step_dev<R>(p, d, prm, nullptr));
+    for (int f = 0; f < 3; ++f) SCL_CHECK(download<R>(p, d.f<R>(f), vi[f]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_initialize3d(int32_t prec, void* vx, void* vy, void* vz, int32_t nx, int32_t ny, int32_t nz,
+                                    int32_t iseed, double akp, double u0, double kcut) {
+  if (!vx || !vy || !vz || nx < 4 || ny < 2 || nz < 2 || (nx & 1)) return set_error("bad initialize arguments"), SCL_EINVAL;
+  if (prec == SCL_SP) return initialize3d<float>((float*)vx, (float*)vy, (float*)vz, nx, ny, nz, iseed, akp, u0, kcut);
+  return initialize3d<double>((double*)vx, (double*)vy, (double*)vz, nx, ny, nz, iseed, akp, u0, kcut);
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident solver
+extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_dns** out) {
+  if (!out || !prm) return set_error("null argument"), SCL_EINVAL;
+  *out = nullptr;
+  SCL_PLAN(p);
+  if (prm->forced < 0 || prm->forced > 2) return set_error("forced must be 0, 1 or 2"), SCL_EINVAL;
+  auto* d = new scl_dns();
+  d->plan = p;
+  d->p = *prm;
+  const size_t bytes = 15 * (size_t)p->fstride() * p->esize();
+  if (hipMalloc(&d->fields, bytes) != hipSuccess) {
+    hipGetLastError();
+    delete d;
+    return set_error("device allocation of %zu bytes for the solver failed", bytes), SCL_ENOMEM;
+  }
+  hipMemsetAsync(d->fields, 0, bytes, p->stream);
+  hipStreamSynchronize(p->stream);
+  *out = d;
+  return SCL_OK;
+}
+
+static void drop_graph(scl_dns* d) {
+  if (d->exec) hipGraphExecDestroy(d->exec);
+  if (d->graph) hipGraphDestroy(d->graph);
+  d->exec = nullptr;
+  d->graph = nullptr;
+}
+
+extern "C" int32_t scl_dns_destroy(scl_dns* d) {
+  if (!d) return SCL_OK;
+  hipStreamSynchronize(d->plan->stream);
+  drop_graph(d);
+  if (d->fields) hipFree(d->fields);
+  delete d;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_dns_set_velocity(scl_dns* d, const void* vx, const void* vy, const void* vz) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  scl_plan3d* p = d->plan;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 0, &s));
+    DevFields f{d->fields, p->fstride()};
+    const void* vi[3] = {vx, vy, vz};
+    for (int q = 0; q < 3; ++q) SCL_CHECK(upload<R>(p, vi[q], f.f<R>(q)));
+    SCL_HIP(hipStreamSynchronize(p->stream));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_dns_get_velocity(scl_dns* d, void* vx, void* vy, void* vz) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  scl_plan3d* p = d->plan;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void* s;
+    SCL_CHECK(plan_scratch(p, 0, &s));
+    DevFields f{d->fields, p->fstride()};
+    void* vo[3] = {vx, vy, vz};
+    for (int q = 0; q < 3; ++q) SCL_CHECK(download<R>(p, f.f<R>(q), vo[q]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_dns_initialize(scl_dns* d, int32_t iseed, double akp, double u0) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  const Geom& g = d->plan->g;
+  const size_t n = (size_t)(g.mx + 1) * g.nrow * 2;
+  if (d->plan->prec == SCL_SP) {
+    std::vector<float> a(n), b(n), c(n);
+    SCL_CHECK(initialize3d<float>(a.data(), b.data(), c.data(), g.nx, g.ny, g.nz, iseed, akp, u0, -1.0));
+    return scl_dns_set_velocity(d, a.data(), b.data(), c.data());
+  }
+  std::vector<double> a(n), b(n), c(n);
+  SCL_CHECK(initialize3d<double>(a.data(), b.data(), c.data(), g.nx, g.ny, g.nz, iseed, akp, u0, -1.0));
+  return scl_dns_set_velocity(d, a.data(), b.data(), c.data());
+}
+
+template <class R>
+static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
+  scl_plan3d* p = d->plan;
+  DevFields f{d->fields, p->fstride()};
+  if (d->p.use_graph && !tm) {
+    if (!d->exec) {
+      SCL_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+      int32_t st = rk4_step_dev<R>(p, f, d->p, nullptr);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(p->stream, &gr);
+      if (st != SCL_OK) {
+        if (gr) hipGraphDestroy(gr);
+        return st;
+      }
+      if (e != hipSuccess) return set_error("graph capture failed: %s", hipGetErrorString(e)), SCL_EHIP;
+      d->graph = gr;
+      SCL_HIP(hipGraphInstantiate(&d->exec, d->graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsteps; ++i) SCL_HIP(hipGraphLaunch(d->exec, p->stream));
+  } else {
+    for (int i = 0; i < nsteps; ++i) SCL_CHECK(rk4_step_dev<R>(p, f, d->p, tm));
+  }
+  d->steps += nsteps;
+  d->time += nsteps * d->p.dt;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_dns_step(scl_dns* d, int32_t nsteps) {
+  if (!d || nsteps < 0) return set_error("bad arguments"), SCL_EINVAL;
+  return by_prec(d->plan->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    return dns_launch_steps<R>(d, nsteps, nullptr);
+  });
+}
+
+extern "C" int32_t scl_dns_sync(scl_dns* d) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  SCL_HIP(hipStreamSynchronize(d->plan->stream));
+  return SCL_OK;
+}
+
+extern "C" double scl_dns_time(const scl_dns* d) { return d ? d->time : 0.0; }
+
+extern "C" int32_t scl_dns_step_timed(scl_dns* d, int32_t nsteps, double* kind_ms, int32_t* kind_count,
+                                      double* total_ms) {
+  if (!d || nsteps < 1) return set_error("bad arguments"), SCL_EINVAL;
+  scl_plan3d* p = d->plan;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kinds;
+  Timer tm{&ev, &kinds, p->stream};
+  int32_t st = by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    return dns_launch_steps<R>(d, nsteps, &tm);
+  });
+  hipStreamSynchronize(p->stream);
+  if (st == SCL_OK) {
+    for (int k = 0; k < SCL_NKIND; ++k) {
+      if (kind_ms) kind_ms[k] = 0.0;
+      if (kind_count) kind_count[k] = 0;
+    }
+    for (size_t i = 0; i + 1 < ev.size(); ++i) {
+      if (kinds[i] < 0) continue;
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      if (kind_ms) kind_ms[kinds[i]] += ms;
+      if (kind_count) kind_count[kinds[i]] += 1;
+    }
+    if (total_ms && ev.size() >= 2) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev.front(), ev.back());
+      *total_ms = ms;
+    }
+  }
+  for (auto e : ev) hipEventDestroy(e);
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics
+template <class R>
+static int32_t diag_run(scl_plan3d* p, const cpx<R>* v, int nbins, std::vector<double>& res) {
+  const Geom& g = p->g;
+  const int len = 2 * nbins + DIAG_TOT;
+  void* buf;
+  SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * len * sizeof(double) + 64, &buf));
+  double* part = (double*)buf;
+  double* fin = part + (size_t)DIAG_BLOCKS * len;
+  const size_t shbytes = (size_t)(4 * 2 * nbins + 256 * 5) * sizeof(double);
+  hipLaunchKernelGGL((k_diag_partial<R>), dim3(DIAG_BLOCKS), dim3(256), shbytes, p->stream, v, p->fstride(), g, nbins,
+                     part);
+  SCL_CHECK(check_launch("diag_partial"));
+  hipLaunchKernelGGL(k_diag_final, dim3((len + 255) / 256), dim3(256), 0, p->stream, part, DIAG_BLOCKS, len, fin);
+  SCL_CHECK(check_launch("diag_final"));
+  res.resize(len);
+  SCL_HIP(hipMemcpyAsync(res.data(), fin, len * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  SCL_HIP(hipStreamSynchronize(p->stream));
+  return SCL_OK;
+}
+
+template <class R>
+static int32_t divmax_run(scl_plan3d* p, const cpx<R>* v, double* out) {
+  void* s;
+  SCL_CHECK(plan_scratch(p, 1, &s));
+  cpx<R>* f = (cpx<R>*)s;
+  const Geom& g = p->g;
+  hipLaunchKernelGGL((k_div_field<R>), dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, p->stream, v,
+                     p->fstride(), f, g);
+  SCL_CHECK(check_launch("div_field"));
+  SCL_CHECK(Ops<R>::c2r3d(*p, f, p->fstride(), 1));
+  void* db;
+  SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * (2 * (g.mx) + DIAG_TOT) * sizeof(double) + 64, &db));
+  unsigned long long* mx = (unsigned long long*)((char*)db + p->diag_bytes - 64);
+  SCL_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long), p->stream));
+  hipLaunchKernelGGL((k_absmax<R>), dim3(256), dim3(256), 0, p->stream, f, g.nmain, mx);
+  SCL_CHECK(check_launch("absmax"));
+  unsigned long long h = 0;
+  SCL_HIP(hipMemcpyAsync(&h, mx, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+  SCL_HIP(hipStreamSynchronize(p->stream));
+  double dv;
+  std::memcpy(&dv, &h, sizeof(dv));
+  *out = dv;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_dns_spectrum(scl_dns* d, double* ek, int32_t nbins) {
+  if (!d || !ek || nbins < 1 || nbins > d->plan->g.mx) return set_error("bad spectrum arguments"), SCL_EINVAL;
+  scl_plan3d* p = d->plan;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    std::vector<double> r;
+    SCL_CHECK(diag_run<R>(p, (const cpx<R>*)d->fields, nbins, r));
+    std::copy(r.begin(), r.begin() + nbins, ek);
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_dns_helicity_spectrum(scl_dns* d, double* hk, int32_t nbins) {
+  if (!d || !hk || nbins < 1 || nbins > d->plan->g.mx) return set_error("bad spectrum arguments"), SCL_EINVAL;
+  scl_plan3d* p = d->plan;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    std::vector<double> r;
+    SCL_CHECK(diag_run<R>(p, (const cpx<R>*)d->fields, nbins, r));
+    std::copy(r.begin() + nbins, r.begin() + 2 * nbins, hk);
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_dns_diagnostics(scl_dns* d, scl_diag* out) {
+  if (!d || !out) return set_error("null argument"), SCL_EINVAL;
+  scl_plan3d* p = d->plan;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    std::vector<double> r;
+    const int nb = p->g.mx;
+    SCL_CHECK(diag_run<R>(p, (const cpx<R>*)d->fields, nb, r));
+    out->energy = r[2 * nb + 0];
+    out->enstrophy = r[2 * nb + 1];
+    out->dissipation = 2.0 * d->p.nu * r[2 * nb + 1];
+    out->helicity = r[2 * nb + 2];
+    out->divrms_spec = std::sqrt(r[2 * nb + 3]);
+    out->nonfinite = r[2 * nb + 4] > 0 ? 1 : 0;
+    return divmax_run<R>(p, (const cpx<R>*)d->fields, &out->divmax);
+  });
+}

@@ -1,0 +1,667 @@
+// HIP kernels of the 3D pseudo-spectral RHS and IF-RK4 step.
+//
+// One RHS evaluation (ratesdudt, dns-deform.f90:259-288) is five passes over
+// HBM, each a batch of 1D FFTs along one axis with the pointwise work fused
+// into the pass that already holds the data:
+//   ZINV  z-inverse of u and of w = i k x u (vorticity built from the loaded
+//         u, never read from HBM): reads 3 fields, writes 6
+//   YINV  y-inverse of the 6 fields, in place
+//   XPASS per x-row: c2r of the 6 fields, u x w (the packed-real cross
+//         product of convec_dns.f90:30-42, times 1/(nx ny nz)), r2c of the 3
+//         products: reads 6 fields, writes 3 in place
+//   YFWD  y-forward of the 3 fields, in place
+//   ZFWD  z-forward + forcing + symmetrize's zeroing + projection + the
+//         IF-RK4 stage update (dns-deform.f90:164-237), reading v, du and the
+//         stage state
+// plus two O(n^2) kernels: the forcing coefficients (a reduction over the
+// few forced modes) and the kx=0-plane Hermitian copy of symmetrize.
+#pragma once
+
+#include "scl_fft.hpp"
+
+namespace scl {
+
+template <class R>
+struct FieldPtrs {
+  cpx<R>* p[6];
+  int n;
+};
+
+__device__ inline float exp_r(float x) { return expf(x); }
+__device__ inline double exp_r(double x) { return exp(x); }
+__device__ inline float expm1_r(float x) { return expm1f(x); }
+__device__ inline double expm1_r(double x) { return expm1(x); }
+
+// ----------------------------------------------------------------------------
+// Column tiles.  A workgroup transforms CW columns along AXIS (1 = y, 2 = z).
+// Main tiles hold CW consecutive x (one 128-byte segment per row for float);
+// tail tiles hold the kx = nx/2 column for CW values of the other index.
+// ----------------------------------------------------------------------------
+struct Tile {
+  int64_t base;  // offset of this thread's column, row 0
+  int64_t rs;    // row stride
+  int kx;        // x index of the column (mx for the kx = nx/2 plane)
+  int oth;       // the other batch index: z for the y-pass, y for the z-pass
+};
+
+template <int CW, int AXIS>
+__device__ inline Tile col_tile(const Geom& g, int wg, int c) {
+  const int xtn = g.mx / CW;
+  Tile t;
+  if constexpr (AXIS == 1) {
+    const int nmain = xtn * g.nz;
+    if (wg < nmain) {
+      const int xt = wg % xtn, z = wg / xtn;
+      t.base = (int64_t)z * g.ny * g.mx + xt * CW + c;
+      t.rs = g.mx;
+      t.kx = xt * CW + c;
+      t.oth = z;
+    } else {
+      const int z = (wg - nmain) * CW + c;
+      t.base = g.nmain + (int64_t)z * g.ny;
+      t.rs = 1;
+      t.kx = g.mx;
+      t.oth = z;
+    }
+  } else {
+    const int nmain = xtn * g.ny;
+    if (wg < nmain) {
+      const int xt = wg % xtn, y = wg / xtn;
+      t.base = (int64_t)y * g.mx + xt * CW + c;
+      t.rs = (int64_t)g.ny * g.mx;
+      t.kx = xt * CW + c;
+      t.oth = y;
+    } else {
+      const int y = (wg - nmain) * CW + c;
+      t.base = g.nmain + y;
+      t.rs = g.ny;
+      t.kx = g.mx;
+      t.oth = y;
+    }
+  }
+  return t;
+}
+
+template <int CW, int AXIS>
+inline int col_grid(const Geom& g) {
+  return AXIS == 1 ? (g.mx / CW) * g.nz + g.nz / CW : (g.mx / CW) * g.ny + g.ny / CW;
+}
+
+template <int CW>
+struct ColIdx {
+  int c;
+  __device__ inline int operator()(int pos) const { return pos * CW + c; }
+};
+
+struct LinIdx {
+  __device__ inline int operator()(int pos) const { return pos; }
+};
+
+// Columns per workgroup: 128-byte row segments, LDS tile <= 128 KB.
+template <class R, int N>
+struct ColCfg {
+  static constexpr int CW = sizeof(R) == 4 ? 16 : 8;
+  static constexpr int E = 16;
+  static constexpr int T = N / E;
+  static constexpr int BS = CW * T;
+};
+
+// Pure batched column FFT, in place, one field per blockIdx.y.
+template <class R, int N, int CW, int AXIS, int DIR>
+__global__ __launch_bounds__(CW*(N / 16)) void k_colfft(cpx<R>* __restrict__ f0, int64_t fstride, Geom g,
+                                                          const cpx<R>* __restrict__ tw, int tws) {
+  constexpr int E = 16, T = N / E;
+  __shared__ cpx<R> lds[N * CW];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  cpx<R>* f = f0 + (int64_t)blockIdx.y * fstride;
+  const Tile t = col_tile<CW, AXIS>(g, blockIdx.x, c);
+  cpx<R> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = f[t.base + (int64_t)(rt + T * m) * t.rs];
+  Line<R, N, E, DIR>::run(v, lds, rt, tw, tws, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) f[t.base + (int64_t)Line<R, N, E, DIR>::out_pos(m, rt) * t.rs] = v[m];
+}
+
+// ZINV: z-inverse of u and of w = i k x u.  s: stage state (3 fields at
+// stride sstride); out: 6 fields (u then w) at stride wstride.
+template <class R, int N, int CW>
+__global__ __launch_bounds__(CW*(N / 16)) void k_zinv_vort(const cpx<R>* __restrict__ s, int64_t sstride,
+                                                             cpx<R>* __restrict__ w, int64_t wstride, Geom g,
+                                                             const cpx<R>* __restrict__ tw, int tws) {
+  constexpr int E = 16, T = N / E;
+  __shared__ cpx<R> lds[N * CW];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 2>(g, blockIdx.x, c);
+  const R kx = (R)t.kx, ky = (R)kwave(t.oth, g.ny);
+  cpx<R> ax[E], ay[E], az[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int64_t o = t.base + (int64_t)(rt + T * m) * t.rs;
+    ax[m] = s[o];
+    ay[m] = s[o + sstride];
+    az[m] = s[o + 2 * sstride];
+  }
+  using L = Line<R, N, E, +1>;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    cpx<R> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const R kz = (R)kwave(rt + T * m, g.nz);
+      // w = i k x u (dns-deform.f90:268-270)
+      if (f == 3) v[m] = mul_i(ky * az[m] - kz * ay[m]);
+      else if (f == 4) v[m] = mul_i(kz * ax[m] - kx * az[m]);
+      else if (f == 5) v[m] = mul_i(kx * ay[m] - ky * ax[m]);
+      else if (f == 0) v[m] = ax[m];
+      else if (f == 1) v[m] = ay[m];
+      else v[m] = az[m];
+    }
+    L::run(v, lds, rt, tw, tws, ColIdx<CW>{c});
+    cpx<R>* o = w + f * wstride;
+#pragma unroll
+    for (int m = 0; m < E; ++m) o[t.base + (int64_t)L::out_pos(m, rt) * t.rs] = v[m];
+  }
+}
+
+// Parameters of the fused ZFWD epilogue.
+template <class R>
+struct RhsParams {
+  R dt;
+  R nu;
+  R fk2;        // fkmax^2 (forcing where k2 < fk2)
+  R kc2;        // 2/3 cut: zero where k2 >= kc2
+  int forced;   // 0 none, 1 force(), 2 force_heli()
+  int dealias;  // 1: zero k2 >= kc2 in the RHS as well
+};
+
+// ZFWD: z-forward of the three transformed products + forcing + the
+// pointwise part of symmetrize + projection (+ optional RHS truncation),
+// then, for MODE 1..4, the IF-RK4 stage update; MODE 0 writes N(u) back
+// into W.  st is the stage state the forcing uses (v for stage 1).
+//   MODE 1 (dns-deform.f90:164-172, 177-179): k = dt N; du = k; S = E (v + k/2)
+//   MODE 2 (:181-196):  k = dt N / E; du += 2k; S = E (v + k/2)
+//   MODE 3 (:198-213):  k = dt N / E; du += 2k; S = E E (v + k)
+//   MODE 4 (:215-237):  k = dt N / E / E; du += k; v = E E (v + du/6); v = 0 where k2 >= kc2
+// with E = exp(-nu k2 dt / 2) (:90).
+template <class R, int N, int CW, int MODE>
+__global__ __launch_bounds__(CW*(N / 16)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V, cpx<R>* DU,
+                                                            cpx<R>* S, const cpx<R>* st, const double* __restrict__ coef,
+                                                            RhsParams<R> p, Geom g, const cpx<R>* __restrict__ tw,
+                                                            int tws) {
+  constexpr int E = 16, T = N / E;
+  __shared__ cpx<R> lds[N * CW];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 2>(g, blockIdx.x, c);
+  cpx<R> nx_[E], ny_[E], nz_[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int64_t o = t.base + (int64_t)(rt + T * m) * t.rs;
+    nx_[m] = W[o];
+    ny_[m] = W[o + fstride];
+    nz_[m] = W[o + 2 * fstride];
+  }
+  using L = Line<R, N, E, -1>;
+  L::run(nx_, lds, rt, tw, tws, ColIdx<CW>{c});
+  L::run(ny_, lds, rt, tw, tws, ColIdx<CW>{c});
+  L::run(nz_, lds, rt, tw, tws, ColIdx<CW>{c});
+
+  const int ikx = t.kx, iky = kwave(t.oth, g.ny);
+  const R kx = (R)ikx, ky = (R)iky;
+  const R tiny = std::numeric_limits<R>::min();
+  const bool plane_zero = (t.oth == g.ny / 2) || (t.kx == g.mx);
+  R fa = 0, fb = 0;
+  if (p.forced) {
+    fa = (R)coef[0];
+    fb = (R)coef[1];
+  }
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int z = L::out_pos(m, rt);
+    const int ikz = kwave(z, g.nz);
+    const R kz = (R)ikz;
+    const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
+    const R k2 = ik2 == 0 ? tiny : (R)ik2;
+    const int64_t o = t.base + (int64_t)z * t.rs;
+    cpx<R> a = nx_[m], b = ny_[m], d = nz_[m];
+    // forcing (force.f90:25-29 / force_heli.f90:44-52)
+    if (p.forced && k2 < p.fk2) {
+      const cpx<R> sx = st[o], sy = st[o + fstride], sz = st[o + 2 * fstride];
+      a = a + fa * sx;
+      b = b + fa * sy;
+      d = d + fa * sz;
+      if (p.forced == 2) {
+        a = a + fb * mul_i(ky * sz - kz * sy);
+        b = b + fb * mul_i(kz * sx - kx * sz);
+        d = d + fb * mul_i(kx * sy - ky * sx);
+      }
+    }
+    // symmetrize.f90:41-48: ky=-ny/2, kz=-nz/2, kx=nx/2 planes and the mean
+    if (plane_zero || z == g.nz / 2 || ik2 == 0) {
+      a = mk<R>(0, 0);
+      b = a;
+      d = a;
+    }
+    // projection.f90:16-30
+    {
+      const R den = kx * kx + ky * ky + kz * kz + tiny;
+      const cpx<R> q = mk<R>((kx * a.x + ky * b.x + kz * d.x) / den, (kx * a.y + ky * b.y + kz * d.y) / den);
+      a = a - kx * q;
+      b = b - ky * q;
+      d = d - kz * q;
+    }
+    if (p.dealias && k2 >= p.kc2) {
+      a = mk<R>(0, 0);
+      b = a;
+      d = a;
+    }
+    if constexpr (MODE == 0) {
+      W[o] = a;
+      W[o + fstride] = b;
+      W[o + 2 * fstride] = d;
+    } else {
+      // E = exp(x), x = -nu k2 dt/2.  E is within 1e-3 of 1 for most modes,
+      // where FP32 spacing (6e-8) makes a rounded E or E*E a biased factor
+      // that compounds every step; E*w is applied as w + expm1(x)*w, exact
+      // to the rounding of the result.
+      const R x = -p.nu * k2 * p.dt / (R)2;
+      const R ef = exp_r(x);
+      const R em1 = expm1_r(x), em2 = expm1_r((R)2 * x);
+      const cpx<R> n3[3] = {a, b, d};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int64_t oq = o + q * fstride;
+        const cpx<R> v = V[oq];
+        if constexpr (MODE == 1) {
+          const cpx<R> k = p.dt * n3[q];
+          DU[oq] = k;
+          const cpx<R> w = v + (R)0.5 * k;
+          S[oq] = w + em1 * w;
+        } else if constexpr (MODE == 2 || MODE == 3) {
+          const cpx<R> k = mk<R>(p.dt * n3[q].x / ef, p.dt * n3[q].y / ef);
+          DU[oq] = DU[oq] + (R)2 * k;
+          if constexpr (MODE == 2) {
+            const cpx<R> w = v + (R)0.5 * k;
+            S[oq] = w + em1 * w;
+          } else {
+            const cpx<R> w = v + k;
+            S[oq] = w + em2 * w;
+          }
+        } else {
+          const cpx<R> k = mk<R>(p.dt * n3[q].x / ef / ef, p.dt * n3[q].y / ef / ef);
+          const cpx<R> du = DU[oq] + k;
+          const cpx<R> w = mk<R>(v.x + du.x / (R)6, v.y + du.y / (R)6);
+          cpx<R> vn = w + em2 * w;
+          if (k2 >= p.kc2) vn = mk<R>(0, 0);
+          V[oq] = vn;
+        }
+      }
+    }
+  }
+}
+
+// Forcing coefficients from the stage state: a reduction over the box of
+// modes that can satisfy k2 < fk2 (force.f90:16-24, force_heli.f90:15-42),
+// accumulated in FP64 in a fixed order.  coef = {a, b, E_f, Omega_f, uw}.
+template <class R>
+__global__ __launch_bounds__(256) void k_force_coef(const cpx<R>* __restrict__ st, int64_t fstride, Geom g,
+                                                    R fk2, int fbox, double eps, double eta, int type,
+                                                    double* __restrict__ coef) {
+  __shared__ double red[3][256];
+  const int nxb = min(fbox, g.mx) + 1;
+  const int nyb = min(2 * fbox + 1, g.ny), nzb = min(2 * fbox + 1, g.nz);
+  const int total = nxb * nyb * nzb;
+  double e = 0, en = 0, uw = 0;
+  const R tiny = std::numeric_limits<R>::min();
+  for (int i = threadIdx.x; i < total; i += 256) {
+    const int ix = i % nxb, jy = (i / nxb) % nyb, jz = i / (nxb * nyb);
+    const int iy = nyb == g.ny ? jy : (jy - fbox + g.ny) % g.ny;
+    const int iz = nzb == g.nz ? jz : (jz - fbox + g.nz) % g.nz;
+    const int kx = ix, ky = kwave(iy, g.ny), kz = kwave(iz, g.nz);
+    const int ik2 = kx * kx + ky * ky + kz * kz;
+    const R k2 = ik2 == 0 ? tiny : (R)ik2;
+    if (!(k2 < fk2)) continue;
+    const int64_t o = ix < g.mx ? ((int64_t)iz * g.ny + iy) * g.mx + ix : g.nmain + (int64_t)iz * g.ny + iy;
+    const cpx<R> sx = st[o], sy = st[o + fstride], sz = st[o + 2 * fstride];
+    const double wgt = ix == 0 ? 0.5 : 1.0;
+    const double X[3][2] = {{sx.x, sx.y}, {sy.x, sy.y}, {sz.x, sz.y}};
+    e += wgt * (X[0][0] * X[0][0] + X[0][1] * X[0][1] + X[1][0] * X[1][0] + X[1][1] * X[1][1] + X[2][0] * X[2][0] +
+                X[2][1] * X[2][1]);
+    if (type == 2) {
+      // w = i k x s in FP64
+      const double k[3] = {(double)kx, (double)ky, (double)kz};
+      double wv[3][2];
+      for (int q = 0; q < 3; ++q) {
+        const int a1 = (q + 1) % 3, a2 = (q + 2) % 3;
+        const double re = k[a1] * X[a2][0] - k[a2] * X[a1][0];
+        const double im = k[a1] * X[a2][1] - k[a2] * X[a1][1];
+        wv[q][0] = -im;
+        wv[q][1] = re;
+      }
+      for (int q = 0; q < 3; ++q) {
+        en += wgt * (wv[q][0] * wv[q][0] + wv[q][1] * wv[q][1]);
+        uw += wgt * 2.0 * (X[q][0] * wv[q][0] + X[q][1] * wv[q][1]);
+      }
+    }
+  }
+  red[0][threadIdx.x] = e;
+  red[1][threadIdx.x] = en;
+  red[2][threadIdx.x] = uw;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+      red[2][threadIdx.x] += red[2][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double E = red[0][0], O = red[1][0], U = red[2][0];
+    if (type == 2) {
+      const double den = 4.0 * E * O - U * U;
+      coef[0] = (4.0 * eps * O - eta * U) / (2.0 * den);
+      coef[1] = (eta * E - eps * U) / den;
+    } else {
+      coef[0] = eps / (2.0 * E);
+      coef[1] = 0.0;
+    }
+    coef[2] = E;
+    coef[3] = O;
+    coef[4] = U;
+  }
+}
+
+// Hermitian copy on the kx = 0 plane (symmetrize.f90:51-61): for kz < 0
+// (z in [nz/2+1, nz-1]) and, on kz = 0, for ky < 0, c(0,y,z) = conj(c(0,-y,-z)).
+template <class R>
+__global__ __launch_bounds__(256) void k_herm_fix(FieldPtrs<R> fp, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n1 = (int64_t)(g.nz / 2 - 1) * g.ny;
+  const int64_t n2 = g.ny / 2 - 1;
+  int y, z, yp, zp;
+  if (i < n1) {
+    z = g.nz / 2 + 1 + (int)(i / g.ny);
+    y = (int)(i % g.ny);
+    zp = g.nz - z;
+    yp = (g.ny - y) % g.ny;
+  } else if (i < n1 + n2) {
+    z = 0;
+    y = g.ny / 2 + 1 + (int)(i - n1);
+    zp = 0;
+    yp = g.ny - y;
+  } else {
+    return;
+  }
+  const int64_t od = ((int64_t)z * g.ny + y) * g.mx, os = ((int64_t)zp * g.ny + yp) * g.mx;
+  for (int f = 0; f < fp.n; ++f) fp.p[f][od] = conj(fp.p[f][os]);
+}
+
+inline int64_t herm_count(const Geom& g) { return (int64_t)(g.nz / 2 - 1) * g.ny + g.ny / 2 - 1; }
+
+// ----------------------------------------------------------------------------
+// x-rows.  A row of nx reals is the main row of M = nx/2 complex (packed
+// x[2n] + i x[2n+1]) and, in spectral form, X[0..M-1] plus X[M] in nyq.
+// ----------------------------------------------------------------------------
+template <class R, int M>
+struct RowCfg {
+  static constexpr int E = 8;
+  static constexpr int TPR = M / 8;
+  static constexpr int BS = TPR >= 256 ? TPR : 256;
+  static constexpr int RPW = BS / TPR;
+  static constexpr int SR = M + 2;  // LDS stride per row
+};
+
+// Builds, from X[0..M] (main row + nyq), Z[k] = E_k + i O_k with
+// E_k = X[k] + conj X[M-k], O_k = (X[k] - conj X[M-k]) exp(+2 pi i k/nx),
+// so that the length-M inverse of Z is x[2n] + i x[2n+1].  Im X[0] and
+// Im X[M] are ignored, as by FFTW's halfcomplex c2r.
+template <class R, int M>
+__device__ inline void row_c2r_prep(cpx<R> (&v)[8], cpx<R>* lrow, const cpx<R>* mainrow, const cpx<R>* nyqp, int rt,
+                                    const cpx<R>* __restrict__ tw, int twsN) {
+  constexpr int T = M / 8;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) lrow[rt + T * m] = mainrow[rt + T * m];
+  if (rt == 0) lrow[M] = *nyqp;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int k = rt + T * m;
+    cpx<R> a = lrow[k], b = lrow[M - k];
+    if (k == 0) {
+      a.y = 0;
+      b.y = 0;
+    }
+    const cpx<R> bc = conj(b);
+    const cpx<R> e = a + bc;
+    const cpx<R> o = cmul<true>(a - bc, tw[k * twsN]);
+    v[m] = e + mul_i(o);
+  }
+  __syncthreads();
+}
+
+// Forward r2c of a row held in registers as packed reals z[n] (slot m at
+// index pos(m)): length-M FFT, then X[k] = (Zk + conj Z_{M-k})/2
+// - (i/2) exp(-2 pi i k/nx) (Zk - conj Z_{M-k}), k = 0..M, stored to the
+// main row and nyq.
+template <class R, int M, class POS>
+__device__ inline void row_r2c_store(cpx<R> (&v)[8], cpx<R>* lrow, cpx<R>* mainrow, cpx<R>* nyqp, int rt,
+                                     const cpx<R>* __restrict__ tw, int twsM, int twsN, const POS& pos) {
+  constexpr int T = M / 8;
+  using L = Line<R, M, 8, -1>;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) lrow[pos(m)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = lrow[rt + T * m];
+  __syncthreads();
+  L::run(v, lrow, rt, tw, twsM, LinIdx{});
+#pragma unroll
+  for (int m = 0; m < 8; ++m) lrow[L::out_pos(m, rt)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int k = rt + T * m;
+    const cpx<R> zk = lrow[k], zc = conj(lrow[(M - k) & (M - 1)]);
+    const cpx<R> s = zk + zc, d = zk - zc;
+    const cpx<R> x = (R)0.5 * (s - mul_i(tw[k * twsN] * d));
+    mainrow[k] = x;
+  }
+  if (rt == 0) {
+    const cpx<R> z0 = lrow[0];
+    *nyqp = mk<R>(z0.x - z0.y, (R)0);
+  }
+  __syncthreads();
+}
+
+template <class R, int M>
+struct InvPos {
+  int rt;
+  __device__ inline int operator()(int m) const { return Line<R, M, 8, +1>::out_pos(m, rt); }
+};
+template <int M>
+struct EntryPos {
+  int rt;
+  __device__ inline int operator()(int m) const { return rt + (M / 8) * m; }
+};
+
+// XPASS: per row, c2r of 6 fields, u x w * scale, r2c of 3, in place into
+// fields 0..2.  Fields at W + f*fstride.
+template <class R, int M>
+__global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_convec(cpx<R>* W, int64_t fstride, Geom g, R scale,
+                                                                 const cpx<R>* __restrict__ tw, int twsM, int twsN) {
+  using C = RowCfg<R, M>;
+  constexpr int T = C::TPR;
+  __shared__ cpx<R> lds[C::RPW * C::SR];
+  const int slot = threadIdx.x / T, rt = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * C::RPW + slot;
+  cpx<R>* lrow = lds + slot * C::SR;
+  using L = Line<R, M, 8, +1>;
+  cpx<R> a[6][8];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    cpx<R>* base = W + f * fstride;
+    row_c2r_prep<R, M>(a[f], lrow, base + row * M, base + g.nmain + row, rt, tw, twsN);
+    L::run(a[f], lrow, rt, tw, twsM, LinIdx{});
+  }
+  // u x w on the packed reals: real and imaginary parts are the even and
+  // odd x points (convec_dns.f90:30-42)
+  cpx<R> p[3][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const cpx<R> ux = a[0][m], uy = a[1][m], uz = a[2][m], wx = a[3][m], wy = a[4][m], wz = a[5][m];
+    p[0][m] = mk<R>((uy.x * wz.x - uz.x * wy.x) * scale, (uy.y * wz.y - uz.y * wy.y) * scale);
+    p[1][m] = mk<R>((uz.x * wx.x - ux.x * wz.x) * scale, (uz.y * wx.y - ux.y * wz.y) * scale);
+    p[2][m] = mk<R>((ux.x * wy.x - uy.x * wx.x) * scale, (ux.y * wy.y - uy.y * wx.y) * scale);
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    cpx<R>* base = W + f * fstride;
+    row_r2c_store<R, M>(p[f], lrow, base + row * M, base + g.nmain + row, rt, tw, twsM, twsN, InvPos<R, M>{rt});
+  }
+}
+
+// Standalone x-direction c2r (one field per blockIdx.y): spectral rows ->
+// packed real rows in main.
+template <class R, int M>
+__global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_c2r(cpx<R>* W, int64_t fstride, Geom g,
+                                                              const cpx<R>* __restrict__ tw, int twsM, int twsN) {
+  using C = RowCfg<R, M>;
+  constexpr int T = C::TPR;
+  __shared__ cpx<R> lds[C::RPW * C::SR];
+  const int slot = threadIdx.x / T, rt = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * C::RPW + slot;
+  cpx<R>* lrow = lds + slot * C::SR;
+  cpx<R>* base = W + (int64_t)blockIdx.y * fstride;
+  using L = Line<R, M, 8, +1>;
+  cpx<R> a[8];
+  row_c2r_prep<R, M>(a, lrow, base + row * M, base + g.nmain + row, rt, tw, twsN);
+  L::run(a, lrow, rt, tw, twsM, LinIdx{});
+#pragma unroll
+  for (int m = 0; m < 8; ++m) base[row * M + L::out_pos(m, rt)] = a[m];
+}
+
+// Standalone x-direction r2c: packed real rows in main -> spectral rows.
+template <class R, int M>
+__global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_r2c(cpx<R>* W, int64_t fstride, Geom g,
+                                                              const cpx<R>* __restrict__ tw, int twsM, int twsN) {
+  using C = RowCfg<R, M>;
+  constexpr int T = C::TPR;
+  __shared__ cpx<R> lds[C::RPW * C::SR];
+  const int slot = threadIdx.x / T, rt = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * C::RPW + slot;
+  cpx<R>* lrow = lds + slot * C::SR;
+  cpx<R>* base = W + (int64_t)blockIdx.y * fstride;
+  cpx<R> a[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) a[m] = base[row * M + rt + T * m];
+  row_r2c_store<R, M>(a, lrow, base + row * M, base + g.nmain + row, rt, tw, twsM, twsN, EntryPos<M>{rt});
+}
+
+// ----------------------------------------------------------------------------
+// Layout conversion and small pointwise kernels (host-array entry points).
+// ----------------------------------------------------------------------------
+// ref (lx1, ly, lz) <-> main + nyq
+template <class R>
+__global__ __launch_bounds__(256) void k_ref_to_split(const cpx<R>* __restrict__ ref, cpx<R>* __restrict__ f, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = g.nrow * (g.mx + 1);
+  if (i >= n) return;
+  const int64_t row = i / (g.mx + 1);
+  const int x = (int)(i % (g.mx + 1));
+  if (x < g.mx) f[row * g.mx + x] = ref[i];
+  else f[g.nmain + row] = ref[i];
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_split_to_ref(const cpx<R>* __restrict__ f, cpx<R>* __restrict__ ref, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = g.nrow * (g.mx + 1);
+  if (i >= n) return;
+  const int64_t row = i / (g.mx + 1);
+  const int x = (int)(i % (g.mx + 1));
+  ref[i] = x < g.mx ? f[row * g.mx + x] : f[g.nmain + row];
+}
+
+// Decode entry i of the split layout into (x, y, z).
+__device__ inline void split_xyz(const Geom& g, int64_t i, int& x, int& y, int& z) {
+  int64_t row;
+  if (i < g.nmain) {
+    row = i / g.mx;
+    x = (int)(i % g.mx);
+  } else {
+    row = i - g.nmain;
+    x = g.mx;
+  }
+  y = (int)(row % g.ny);
+  z = (int)(row / g.ny);
+}
+
+// symmetrize.f90:41-48 zeroing (the Hermitian copy is k_herm_fix), and,
+// when thr > 0, dealiasing.f90:35's k2 >= thr truncation.
+template <class R>
+__global__ __launch_bounds__(256) void k_sym_zero(cpx<R>* __restrict__ f, Geom g, R thr) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z;
+  split_xyz(g, i, x, y, z);
+  const int kx = x, ky = kwave(y, g.ny), kz = kwave(z, g.nz);
+  const int ik2 = kx * kx + ky * ky + kz * kz;
+  const R k2 = ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2;
+  if (x == g.mx || y == g.ny / 2 || z == g.nz / 2 || ik2 == 0 || (thr > 0 && k2 >= thr)) f[i] = mk<R>(0, 0);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_dealias_mask(cpx<R>* __restrict__ f, Geom g, R thr) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z;
+  split_xyz(g, i, x, y, z);
+  const int kx = x, ky = kwave(y, g.ny), kz = kwave(z, g.nz);
+  const int ik2 = kx * kx + ky * ky + kz * kz;
+  const R k2 = ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2;
+  if (k2 >= thr) f[i] = mk<R>(0, 0);
+}
+
+// projection.f90:16-30 on three fields at stride fstride
+template <class R>
+__global__ __launch_bounds__(256) void k_projection(cpx<R>* __restrict__ v, int64_t fstride, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z;
+  split_xyz(g, i, x, y, z);
+  const R kx = (R)x, ky = (R)kwave(y, g.ny), kz = (R)kwave(z, g.nz);
+  cpx<R> a = v[i], b = v[i + fstride], d = v[i + 2 * fstride];
+  const R den = kx * kx + ky * ky + kz * kz + std::numeric_limits<R>::min();
+  const cpx<R> q = mk<R>((kx * a.x + ky * b.x + kz * d.x) / den, (kx * a.y + ky * b.y + kz * d.y) / den);
+  v[i] = a - kx * q;
+  v[i + fstride] = b - ky * q;
+  v[i + 2 * fstride] = d - kz * q;
+}
+
+// force (type 1): w += a v on k2 < fk2;  force_heli (type 2): f = a v + b w
+// on k2 < fk2, 0 elsewhere and at k = 0 (written into w's fields).
+template <class R>
+__global__ __launch_bounds__(256) void k_force_apply(cpx<R>* __restrict__ w, const cpx<R>* __restrict__ v,
+                                                     int64_t fstride, Geom g, const double* __restrict__ coef, R fk2,
+                                                     int type) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z;
+  split_xyz(g, i, x, y, z);
+  const int ky = kwave(y, g.ny), kz = kwave(z, g.nz);
+  const int ik2 = x * x + ky * ky + kz * kz;
+  const R k2 = ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2;
+  const R a = (R)coef[0], b = (R)coef[1];
+  const bool in = k2 < fk2;
+  for (int q = 0; q < 3; ++q) {
+    const int64_t o = i + q * fstride;
+    if (type == 1) {
+      if (in) w[o] = w[o] + a * v[o];
+    } else {
+      w[o] = (in && ik2 != 0) ? a * v[o] + b * w[o] : mk<R>(0, 0);
+    }
+  }
+}
+
+}  // namespace scl

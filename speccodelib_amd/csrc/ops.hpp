@@ -1,0 +1,41 @@
+// Typed launchers: runtime sizes -> kernel template instances.
+#pragma once
+
+#include <type_traits>
+
+#include "kernels3d.hpp"
+#include "plan.hpp"
+
+namespace scl {
+
+template <class R>
+struct Ops {
+  using C = cpx<R>;
+  static int32_t colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int axis, int dir);
+  static int32_t zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* w, int64_t wstride);
+  static int32_t zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
+                          const double* coef, const RhsParams<R>& p);
+  static int32_t force_coef(const scl_plan3d& pl, const C* st, int64_t fstride, double fkmax, double eps, double eta,
+                            int type, double* coef);
+  static int32_t herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp);
+  static int32_t row_convec(const scl_plan3d& pl, C* W, int64_t fstride);
+  static int32_t row_c2r(const scl_plan3d& pl, C* W, int64_t fstride, int nf);
+  static int32_t row_r2c(const scl_plan3d& pl, C* W, int64_t fstride, int nf);
+  static int32_t ref_to_split(const scl_plan3d& pl, const C* ref, C* f);
+  static int32_t split_to_ref(const scl_plan3d& pl, const C* f, C* ref);
+  static int32_t sym_zero(const scl_plan3d& pl, C* f, R thr);
+  static int32_t dealias_mask(const scl_plan3d& pl, C* f, R thr);
+  static int32_t projection(const scl_plan3d& pl, C* v, int64_t fstride);
+  static int32_t force_apply(const scl_plan3d& pl, C* w, const C* v, int64_t fstride, const double* coef, R fk2,
+                             int type);
+  // full 3D transforms on fields in the split layout, in place
+  static int32_t c2r3d(const scl_plan3d& pl, C* f0, int64_t fstride, int nf);
+  static int32_t r2c3d(const scl_plan3d& pl, C* f0, int64_t fstride, int nf);
+};
+
+extern template struct Ops<float>;
+extern template struct Ops<double>;
+
+int32_t check_launch(const char* what);
+
+}  // namespace scl

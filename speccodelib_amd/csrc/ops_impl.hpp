@@ -1,0 +1,261 @@
+// Definitions of Ops<R>; instantiated once per precision (ops_sp.hip,
+// ops_dp.hip) so the two precisions compile in parallel.
+#pragma once
+
+#include "ops.hpp"
+
+namespace scl {
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// Column/line lengths supported per precision.
+template <class R>
+constexpr int max_line() {
+  return sizeof(R) == 4 ? 1024 : 256;
+}
+
+template <class R, class F>
+int32_t with_line(int n, F&& f) {
+  switch (n) {
+    case 16: return f(IC<16>{});
+    case 32: return f(IC<32>{});
+    case 64: return f(IC<64>{});
+    case 128: return f(IC<128>{});
+    case 256: return f(IC<256>{});
+    case 512:
+      if constexpr (max_line<R>() >= 512) return f(IC<512>{});
+      break;
+    case 1024:
+      if constexpr (max_line<R>() >= 1024) return f(IC<1024>{});
+      break;
+    default: break;
+  }
+  set_error("unsupported line length %d for this precision", n);
+  return SCL_EINVAL;
+}
+
+template <class R, class F>
+int32_t with_half(int m, F&& f) {  // m = nx/2
+  switch (m) {
+    case 8: return f(IC<8>{});
+    case 16: return f(IC<16>{});
+    case 32: return f(IC<32>{});
+    case 64: return f(IC<64>{});
+    case 128: return f(IC<128>{});
+    case 256:
+      if constexpr (max_line<R>() >= 512) return f(IC<256>{});
+      break;
+    case 512:
+      if constexpr (max_line<R>() >= 1024) return f(IC<512>{});
+      break;
+    default: break;
+  }
+  set_error("unsupported nx/2 = %d for this precision", m);
+  return SCL_EINVAL;
+}
+
+template <class R>
+static inline const cpx<R>* twp(const scl_plan3d& pl) {
+  return reinterpret_cast<const cpx<R>*>(pl.tw);
+}
+
+static inline dim3 grid1(int64_t n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+template <class R>
+int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int axis, int dir) {
+  const Geom& g = pl.g;
+  const int n = axis == 1 ? g.ny : g.nz;
+  return with_line<R>(n, [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    using CC = ColCfg<R, N>;
+    constexpr int CW = (CC::CW > 8) ? CC::CW : 8;
+    if (g.mx < CW) {
+      // narrow grids: 8 columns per workgroup
+      constexpr int CW8 = 8;
+      dim3 blk(CW8 * (N / 16));
+      if (axis == 1) {
+        dim3 grd(col_grid<CW8, 1>(g), nf);
+        if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW8, 1, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+        else hipLaunchKernelGGL((k_colfft<R, N, CW8, 1, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+      } else {
+        dim3 grd(col_grid<CW8, 2>(g), nf);
+        if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW8, 2, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+        else hipLaunchKernelGGL((k_colfft<R, N, CW8, 2, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+      }
+      return check_launch("colfft8");
+    }
+    dim3 blk(CW * (N / 16));
+    if (axis == 1) {
+      dim3 grd(col_grid<CW, 1>(g), nf);
+      if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW, 1, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+      else hipLaunchKernelGGL((k_colfft<R, N, CW, 1, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+    } else {
+      dim3 grd(col_grid<CW, 2>(g), nf);
+      if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW, 2, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+      else hipLaunchKernelGGL((k_colfft<R, N, CW, 2, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+    }
+    return check_launch("colfft");
+  });
+}
+
+// Columns per workgroup actually used for a plan (see colfft).
+template <class R, int N>
+constexpr int cw_for(int mx) {
+  return ColCfg<R, N>::CW;
+}
+
+template <class R>
+int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* w, int64_t wstride) {
+  const Geom& g = pl.g;
+  return with_line<R>(g.nz, [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    constexpr int CW = ColCfg<R, N>::CW;
+    if (g.mx < CW) {
+      hipLaunchKernelGGL((k_zinv_vort<R, N, 8>), dim3(col_grid<8, 2>(g)), dim3(8 * (N / 16)), 0, pl.stream, s,
+                         sstride, w, wstride, g, twp<R>(pl), pl.L / N);
+    } else {
+      hipLaunchKernelGGL((k_zinv_vort<R, N, CW>), dim3(col_grid<CW, 2>(g)), dim3(CW * (N / 16)), 0, pl.stream, s,
+                         sstride, w, wstride, g, twp<R>(pl), pl.L / N);
+    }
+    return check_launch("zinv_vort");
+  });
+}
+
+template <class R>
+int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
+                         const double* coef, const RhsParams<R>& p) {
+  const Geom& g = pl.g;
+  return with_line<R>(g.nz, [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    constexpr int CWF = ColCfg<R, N>::CW;
+    auto go = [&](auto CWC) -> int32_t {
+      constexpr int CW = decltype(CWC)::value;
+      dim3 grd(col_grid<CW, 2>(g)), blk(CW * (N / 16));
+      const cpx<R>* tw = twp<R>(pl);
+      const int tws = pl.L / N;
+      switch (mode) {
+        case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 0>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 1>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 2>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 3>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 4>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        default: set_error("bad mode %d", mode); return SCL_EINVAL;
+      }
+      return check_launch("zfwd_rhs");
+    };
+    if (g.mx < CWF) return go(IC<8>{});
+    return go(IC<CWF>{});
+  });
+}
+
+template <class R>
+int32_t Ops<R>::force_coef(const scl_plan3d& pl, const C* st, int64_t fstride, double fkmax, double eps, double eta,
+                           int type, double* coef) {
+  const R fk = (R)fkmax;
+  const R fk2 = fk * fk;
+  int fbox = (int)std::floor(fkmax);
+  if (fbox < 0) fbox = 0;
+  hipLaunchKernelGGL((k_force_coef<R>), dim3(1), dim3(256), 0, pl.stream, st, fstride, pl.g, fk2, fbox, eps, eta, type,
+                     coef);
+  return check_launch("force_coef");
+}
+
+template <class R>
+int32_t Ops<R>::herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp) {
+  const int64_t n = herm_count(pl.g);
+  if (n <= 0) return SCL_OK;
+  hipLaunchKernelGGL((k_herm_fix<R>), grid1(n), dim3(256), 0, pl.stream, fp, pl.g);
+  return check_launch("herm_fix");
+}
+
+template <class R>
+int32_t Ops<R>::row_convec(const scl_plan3d& pl, C* W, int64_t fstride) {
+  const Geom& g = pl.g;
+  const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
+  return with_half<R>(g.mx, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    using RC = RowCfg<R, M>;
+    hipLaunchKernelGGL((k_row_convec<R, M>), dim3((unsigned)(g.nrow / RC::RPW)), dim3(RC::BS), 0, pl.stream, W,
+                       fstride, g, scale, twp<R>(pl), pl.L / M, pl.L / g.nx);
+    return check_launch("row_convec");
+  });
+}
+
+template <class R>
+int32_t Ops<R>::row_c2r(const scl_plan3d& pl, C* W, int64_t fstride, int nf) {
+  const Geom& g = pl.g;
+  return with_half<R>(g.mx, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    using RC = RowCfg<R, M>;
+    hipLaunchKernelGGL((k_row_c2r<R, M>), dim3((unsigned)(g.nrow / RC::RPW), nf), dim3(RC::BS), 0, pl.stream, W,
+                       fstride, g, twp<R>(pl), pl.L / M, pl.L / g.nx);
+    return check_launch("row_c2r");
+  });
+}
+
+template <class R>
+int32_t Ops<R>::row_r2c(const scl_plan3d& pl, C* W, int64_t fstride, int nf) {
+  const Geom& g = pl.g;
+  return with_half<R>(g.mx, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    using RC = RowCfg<R, M>;
+    hipLaunchKernelGGL((k_row_r2c<R, M>), dim3((unsigned)(g.nrow / RC::RPW), nf), dim3(RC::BS), 0, pl.stream, W,
+                       fstride, g, twp<R>(pl), pl.L / M, pl.L / g.nx);
+    return check_launch("row_r2c");
+  });
+}
+
+template <class R>
+int32_t Ops<R>::ref_to_split(const scl_plan3d& pl, const C* ref, C* f) {
+  hipLaunchKernelGGL((k_ref_to_split<R>), grid1(pl.g.nrow * (pl.g.mx + 1)), dim3(256), 0, pl.stream, ref, f, pl.g);
+  return check_launch("ref_to_split");
+}
+
+template <class R>
+int32_t Ops<R>::split_to_ref(const scl_plan3d& pl, const C* f, C* ref) {
+  hipLaunchKernelGGL((k_split_to_ref<R>), grid1(pl.g.nrow * (pl.g.mx + 1)), dim3(256), 0, pl.stream, f, ref, pl.g);
+  return check_launch("split_to_ref");
+}
+
+template <class R>
+int32_t Ops<R>::sym_zero(const scl_plan3d& pl, C* f, R thr) {
+  hipLaunchKernelGGL((k_sym_zero<R>), grid1(pl.g.ntot), dim3(256), 0, pl.stream, f, pl.g, thr);
+  return check_launch("sym_zero");
+}
+
+template <class R>
+int32_t Ops<R>::dealias_mask(const scl_plan3d& pl, C* f, R thr) {
+  hipLaunchKernelGGL((k_dealias_mask<R>), grid1(pl.g.ntot), dim3(256), 0, pl.stream, f, pl.g, thr);
+  return check_launch("dealias_mask");
+}
+
+template <class R>
+int32_t Ops<R>::projection(const scl_plan3d& pl, C* v, int64_t fstride) {
+  hipLaunchKernelGGL((k_projection<R>), grid1(pl.g.ntot), dim3(256), 0, pl.stream, v, fstride, pl.g);
+  return check_launch("projection");
+}
+
+template <class R>
+int32_t Ops<R>::force_apply(const scl_plan3d& pl, C* w, const C* v, int64_t fstride, const double* coef, R fk2,
+                            int type) {
+  hipLaunchKernelGGL((k_force_apply<R>), grid1(pl.g.ntot), dim3(256), 0, pl.stream, w, v, fstride, pl.g, coef, fk2,
+                     type);
+  return check_launch("force_apply");
+}
+
+template <class R>
+int32_t Ops<R>::c2r3d(const scl_plan3d& pl, C* f0, int64_t fstride, int nf) {
+  SCL_CHECK(colfft(pl, f0, fstride, nf, 2, +1));
+  SCL_CHECK(colfft(pl, f0, fstride, nf, 1, +1));
+  return row_c2r(pl, f0, fstride, nf);
+}
+
+template <class R>
+int32_t Ops<R>::r2c3d(const scl_plan3d& pl, C* f0, int64_t fstride, int nf) {
+  SCL_CHECK(row_r2c(pl, f0, fstride, nf));
+  SCL_CHECK(colfft(pl, f0, fstride, nf, 1, -1));
+  return colfft(pl, f0, fstride, nf, 2, -1);
+}
+
+}  // namespace scl

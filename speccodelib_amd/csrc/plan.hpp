@@ -1,0 +1,42 @@
+// Host-side plan and solver objects behind the opaque C handles.
+#pragma once
+
+#include <vector>
+
+#include "scl_core.hpp"
+
+struct scl_plan3d {
+  int prec = SCL_SP;
+  scl::Geom g{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* tw = nullptr;  // twiddle table exp(-2 pi i j/L), j < L, in the plan's precision
+  int L = 0;
+  void* scratch = nullptr;  // host-array entry points: 7 fields (6 work + 1 staging)
+  size_t scratch_fields = 0;
+  double* coef = nullptr;     // forcing coefficients (8 doubles)
+  void* diag = nullptr;       // diagnostics partial sums
+  size_t diag_bytes = 0;
+  size_t esize() const { return prec == SCL_SP ? 2 * sizeof(float) : 2 * sizeof(double); }
+  // complex entries of one field in the split device layout
+  int64_t fstride() const { return g.ntot; }
+  // bytes of one field in the reference layout (lx1, ly, lz)
+  size_t ref_bytes() const { return (size_t)(g.mx + 1) * g.nrow * esize(); }
+};
+
+namespace scl {
+int32_t plan_scratch(scl_plan3d* p, size_t nfields, void** out);
+int32_t plan_diag(scl_plan3d* p, size_t bytes, void** out);
+}  // namespace scl
+
+struct scl_dns {
+  scl_plan3d* plan = nullptr;
+  scl_dns_params p{};
+  void* fields = nullptr;  // V(3) DU(3) S(3) W(6), each plan->fstride() entries
+  double time = 0.0;
+  int64_t steps = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipEvent_t> events;
+};

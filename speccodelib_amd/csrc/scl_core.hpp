@@ -1,0 +1,110 @@
+// Core types shared by the HIP kernels and the C-ABI layer.
+//
+// Device layout of one complex half-spectrum field (the reference's
+// complex(sp), dimension(lx1, ly, lz) with lx1 = nx/2 + 1):
+//   main[z][y][x], x in [0, nx/2)   row pitch nx/2 complex (rows 128-byte
+//                                   aligned for nx >= 32)
+//   nyq [z][y]                      the kx = nx/2 plane, stored apart so the
+//                                   main rows stay aligned
+// Both parts live in one allocation: main first, nyq right after it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/scl.h"
+
+namespace scl {
+
+template <class R>
+struct alignas(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <class R>
+__host__ __device__ inline cpx<R> mk(R a, R b) {
+  cpx<R> c;
+  c.x = a;
+  c.y = b;
+  return c;
+}
+template <class R>
+__host__ __device__ inline cpx<R> operator+(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.x + b.x, a.y + b.y);
+}
+template <class R>
+__host__ __device__ inline cpx<R> operator-(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.x - b.x, a.y - b.y);
+}
+template <class R>
+__host__ __device__ inline cpx<R> operator*(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class R>
+__host__ __device__ inline cpx<R> operator*(R s, cpx<R> a) {
+  return mk<R>(s * a.x, s * a.y);
+}
+template <class R>
+__host__ __device__ inline cpx<R> conj(cpx<R> a) {
+  return mk<R>(a.x, -a.y);
+}
+// i * a
+template <class R>
+__host__ __device__ inline cpx<R> mul_i(cpx<R> a) {
+  return mk<R>(-a.y, a.x);
+}
+// a * conj(b) when CONJ, else a * b
+template <bool CONJ, class R>
+__host__ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  if (CONJ) return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return a * b;
+}
+
+// Geometry of a 3D plan, shared by all kernels.
+struct Geom {
+  int nx, ny, nz;  // physical grid
+  int mx;          // nx/2: complex entries per main row
+  int64_t nrow;    // ny*nz rows
+  int64_t nmain;   // mx*ny*nz complex entries in main
+  int64_t ntot;    // nmain + ny*nz (main + nyq)
+};
+
+inline Geom make_geom(int nx, int ny, int nz) {
+  Geom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.mx = nx / 2;
+  g.nrow = (int64_t)ny * nz;
+  g.nmain = (int64_t)g.mx * g.nrow;
+  g.ntot = g.nmain + g.nrow;
+  return g;
+}
+
+// Signed wavenumber of index i on an axis of length n (wavenumber.f90:109).
+__host__ __device__ inline int kwave(int i, int n) { return i < n / 2 ? i : i - n; }
+
+// Error state (thread-local message, read by scl_last_error()).
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define SCL_HIP(call)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      scl::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,                 \
+                     hipGetErrorString(e_));                                      \
+      return SCL_EHIP;                                                            \
+    }                                                                             \
+  } while (0)
+
+#define SCL_CHECK(expr)          \
+  do {                           \
+    int32_t s_ = (expr);         \
+    if (s_ != SCL_OK) return s_; \
+  } while (0)
+
+}  // namespace scl
