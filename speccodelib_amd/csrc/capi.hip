@@ -232,6 +232,11 @@ extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t
   p->prec = prec;
   p->g = make_geom(nx, ny, nz);
   hipGetDevice(&p->device);
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess && ncu > 0)
+      p->ncu = ncu;
+  }
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     delete p;
     return set_error("stream creation failed"), SCL_EHIP;

@@ -87,80 +87,123 @@ inline int col_grid(const Geom& g) {
   return AXIS == 1 ? (g.mx / CW) * g.nz + g.nz / CW : (g.mx / CW) * g.ny + g.ny / CW;
 }
 
+// LDS addressing of the Stockham exchanges.  One padding slot after every
+// 8 entries (pos + pos/8) keeps the stride-8 writes of a radix-8 pass and
+// the unit-stride reads bank-conflict-free within each 16-lane group.
+__host__ __device__ constexpr int lds_pad(int n) { return n + n / 8; }
+
 template <int CW>
 struct ColIdx {
   int c;
-  __device__ inline int operator()(int pos) const { return pos * CW + c; }
+  __device__ inline int operator()(int pos) const { return (pos + (pos >> 3)) * CW + c; }
 };
 
 struct LinIdx {
-  __device__ inline int operator()(int pos) const { return pos; }
+  __device__ inline int operator()(int pos) const { return pos + (pos >> 3); }
 };
 
-// Columns per workgroup: 128-byte row segments, LDS tile <= 128 KB.
+// Column tile shape: CW columns of N points, E points per thread.  CW = 8
+// gives 64-byte row segments; the two halves of a 128-byte line belong to
+// logical tiles t and t+1, which xcd_swizzle() places on the same XCD (one
+// L2) at nearly the same time.
+#ifndef SCL_COL_CW
+#define SCL_COL_CW 8
+#endif
 template <class R, int N>
 struct ColCfg {
-  static constexpr int CW = sizeof(R) == 4 ? 16 : 8;
-  static constexpr int E = 16;
+  static constexpr int CW = SCL_COL_CW;
+  static constexpr int E = N <= 512 ? 8 : 16;
   static constexpr int T = N / E;
   static constexpr int BS = CW * T;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD
+// b mod 8).  Map block b to logical tile (b mod 8) * (G/8) + b/8 so that each
+// XCD walks a contiguous range of tiles; the last G mod 8 blocks keep their
+// index.
+__device__ inline int xcd_swizzle(int b, int G) {
+  const int G8 = G & ~7;
+  if (b >= G8) return b;
+  return (b & 7) * (G8 >> 3) + (b >> 3);
+}
+
 // Pure batched column FFT, in place, one field per blockIdx.y.
-template <class R, int N, int CW, int AXIS, int DIR>
-__global__ __launch_bounds__(CW*(N / 16)) void k_colfft(cpx<R>* __restrict__ f0, int64_t fstride, Geom g,
-                                                          const cpx<R>* __restrict__ tw, int tws) {
-  constexpr int E = 16, T = N / E;
-  __shared__ cpx<R> lds[N * CW];
+template <class R, int N, int AXIS, int DIR>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restrict__ f0, int64_t fstride, Geom g,
+                                                              const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   cpx<R>* f = f0 + (int64_t)blockIdx.y * fstride;
-  const Tile t = col_tile<CW, AXIS>(g, blockIdx.x, c);
+  const Tile t = col_tile<CW, AXIS>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
   cpx<R> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = f[t.base + (int64_t)(rt + T * m) * t.rs];
-  Line<R, N, E, DIR>::run(v, lds, rt, tw, tws, ColIdx<CW>{c});
+  Line<R, N, E, DIR>::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
   for (int m = 0; m < E; ++m) f[t.base + (int64_t)Line<R, N, E, DIR>::out_pos(m, rt) * t.rs] = v[m];
 }
 
 // ZINV: z-inverse of u and of w = i k x u.  s: stage state (3 fields at
 // stride sstride); out: 6 fields (u then w) at stride wstride.
-template <class R, int N, int CW>
-__global__ __launch_bounds__(CW*(N / 16)) void k_zinv_vort(const cpx<R>* __restrict__ s, int64_t sstride,
-                                                             cpx<R>* __restrict__ w, int64_t wstride, Geom g,
-                                                             const cpx<R>* __restrict__ tw, int tws) {
-  constexpr int E = 16, T = N / E;
-  __shared__ cpx<R> lds[N * CW];
+template <class R, int N>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_zinv_vort(const cpx<R>* __restrict__ s, int64_t sstride,
+                                                                 cpx<R>* __restrict__ w, int64_t wstride, Geom g,
+                                                                 const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
-  const Tile t = col_tile<CW, 2>(g, blockIdx.x, c);
+  const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
   const R kx = (R)t.kx, ky = (R)kwave(t.oth, g.ny);
-  cpx<R> ax[E], ay[E], az[E];
+  using L = Line<R, N, E, +1>;
+  // Row of slot m, on entry (kz index) and after the transform (z index):
+  // the last Stockham pass leaves slot m at index rt + T*m.
+  int64_t off[E];
+  R kz[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int64_t o = t.base + (int64_t)(rt + T * m) * t.rs;
-    ax[m] = s[o];
-    ay[m] = s[o + sstride];
-    az[m] = s[o + 2 * sstride];
+    off[m] = t.base + (int64_t)(rt + T * m) * t.rs;
+    kz[m] = (R)kwave(rt + T * m, g.nz);
   }
-  using L = Line<R, N, E, +1>;
+  // w = i k x u (dns-deform.f90:268-270), by linearity of the z-transform Z:
+  //   Z(wx) = i (ky Uz - Z(kz uy)),  Z(wy) = i (Z(kz ux) - kx Uz),
+  //   Z(wz) = i (kx Uy - ky Ux),     Uq = Z(uq): five transforms, not six.
+  cpx<R> uz[E], uy[E], ux[E], d[E];
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    cpx<R> v[E];
+  for (int m = 0; m < E; ++m) uz[m] = s[off[m] + 2 * sstride];
+  L::run(uz, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const R kz = (R)kwave(rt + T * m, g.nz);
-      // w = i k x u (dns-deform.f90:268-270)
-      if (f == 3) v[m] = mul_i(ky * az[m] - kz * ay[m]);
-      else if (f == 4) v[m] = mul_i(kz * ax[m] - kx * az[m]);
-      else if (f == 5) v[m] = mul_i(kx * ay[m] - ky * ax[m]);
-      else if (f == 0) v[m] = ax[m];
-      else if (f == 1) v[m] = ay[m];
-      else v[m] = az[m];
-    }
-    L::run(v, lds, rt, tw, tws, ColIdx<CW>{c});
-    cpx<R>* o = w + f * wstride;
+  for (int m = 0; m < E; ++m) w[off[m] + 2 * wstride] = uz[m];
 #pragma unroll
-    for (int m = 0; m < E; ++m) o[t.base + (int64_t)L::out_pos(m, rt) * t.rs] = v[m];
+  for (int m = 0; m < E; ++m) {
+    uy[m] = s[off[m] + sstride];
+    d[m] = kz[m] * uy[m];
+  }
+  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + 3 * wstride] = mul_i(ky * uz[m] - d[m]);
+  L::run(uy, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + wstride] = uy[m];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    ux[m] = s[off[m]];
+    d[m] = kz[m] * ux[m];
+  }
+  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + 4 * wstride] = mul_i(d[m] - kx * uz[m]);
+  L::run(ux, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    w[off[m]] = ux[m];
+    w[off[m] + 5 * wstride] = mul_i(kx * uy[m] - ky * ux[m]);
   }
 }
 
@@ -184,15 +227,18 @@ struct RhsParams {
 //   MODE 3 (:198-213):  k = dt N / E; du += 2k; S = E E (v + k)
 //   MODE 4 (:215-237):  k = dt N / E / E; du += k; v = E E (v + du/6); v = 0 where k2 >= kc2
 // with E = exp(-nu k2 dt / 2) (:90).
-template <class R, int N, int CW, int MODE>
-__global__ __launch_bounds__(CW*(N / 16)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V, cpx<R>* DU,
-                                                            cpx<R>* S, const cpx<R>* st, const double* __restrict__ coef,
-                                                            RhsParams<R> p, Geom g, const cpx<R>* __restrict__ tw,
-                                                            int tws) {
-  constexpr int E = 16, T = N / E;
-  __shared__ cpx<R> lds[N * CW];
+template <class R, int N, int MODE>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V, cpx<R>* DU,
+                                                                cpx<R>* S, const cpx<R>* st,
+                                                                const double* __restrict__ coef, RhsParams<R> p, Geom g,
+                                                                const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
-  const Tile t = col_tile<CW, 2>(g, blockIdx.x, c);
+  const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
   cpx<R> nx_[E], ny_[E], nz_[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -202,9 +248,9 @@ __global__ __launch_bounds__(CW*(N / 16)) void k_zfwd_rhs(cpx<R>* W, int64_t fst
     nz_[m] = W[o + 2 * fstride];
   }
   using L = Line<R, N, E, -1>;
-  L::run(nx_, lds, rt, tw, tws, ColIdx<CW>{c});
-  L::run(ny_, lds, rt, tw, tws, ColIdx<CW>{c});
-  L::run(nz_, lds, rt, tw, tws, ColIdx<CW>{c});
+  L::run(nx_, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(ny_, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(nz_, lds, rt, twl, 1, ColIdx<CW>{c});
 
   const int ikx = t.kx, iky = kwave(t.oth, g.ny);
   const R kx = (R)ikx, ky = (R)iky;
@@ -403,32 +449,59 @@ inline int64_t herm_count(const Geom& g) { return (int64_t)(g.nz / 2 - 1) * g.ny
 // x-rows.  A row of nx reals is the main row of M = nx/2 complex (packed
 // x[2n] + i x[2n+1]) and, in spectral form, X[0..M-1] plus X[M] in nyq.
 // ----------------------------------------------------------------------------
+// T = M/8 threads own a row (T <= 64, so a row never spans two wavefronts);
+// thread rt holds points rt + T*m.  The last Stockham pass leaves its
+// outputs in that same layout (slot m = index rt + T*m), so an inverse
+// transform feeds the pointwise product and the forward transform with no
+// data exchange, and the index pairs (k, M-k) of the real-FFT packing are
+// one lane shuffle apart.
 template <class R, int M>
 struct RowCfg {
   static constexpr int E = 8;
   static constexpr int TPR = M / 8;
-  static constexpr int BS = TPR >= 256 ? TPR : 256;
+#ifndef SCL_ROW_BS
+#define SCL_ROW_BS 64
+#endif
+  // one wavefront per workgroup: a row's exchanges never wait on other waves
+  static constexpr int BS = SCL_ROW_BS;
   static constexpr int RPW = BS / TPR;
-  static constexpr int SR = M + 2;  // LDS stride per row
 };
 
-// Builds, from X[0..M] (main row + nyq), Z[k] = E_k + i O_k with
-// E_k = X[k] + conj X[M-k], O_k = (X[k] - conj X[M-k]) exp(+2 pi i k/nx),
-// so that the length-M inverse of Z is x[2n] + i x[2n+1].  Im X[0] and
-// Im X[M] are ignored, as by FFTW's halfcomplex c2r.
+template <class R>
+__device__ inline cpx<R> shfl_c(cpx<R> v, int src, int width) {
+  return mk<R>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+
+// Entry at index (M - k) mod M for k = rt + T*m of a row in the entry
+// layout: lane T - rt, slot 7 - m when rt > 0; own slot (8 - m) mod 8 when
+// rt = 0.  Every lane of the row must call it (it shuffles).
 template <class R, int M>
-__device__ inline void row_c2r_prep(cpx<R> (&v)[8], cpx<R>* lrow, const cpx<R>* mainrow, const cpx<R>* nyqp, int rt,
-                                    const cpx<R>* __restrict__ tw, int twsN) {
+__device__ inline cpx<R> row_partner(const cpx<R> (&v)[8], int m, int rt) {
   constexpr int T = M / 8;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) lrow[rt + T * m] = mainrow[rt + T * m];
-  if (rt == 0) lrow[M] = *nyqp;
-  __syncthreads();
+  if constexpr (T == 1) {
+    return v[(8 - m) & 7];
+  } else {
+    const cpx<R> s = shfl_c(v[7 - m], (T - rt) & (T - 1), T);
+    return rt == 0 ? v[(8 - m) & 7] : s;
+  }
+}
+
+// From X[0..M] (x: X[rt + T*m]; xm: X[M], read by rt = 0), builds
+// Z[k] = E_k + i O_k, E_k = X[k] + conj X[M-k],
+// O_k = (X[k] - conj X[M-k]) exp(+2 pi i k/nx), whose length-M inverse is
+// x[2n] + i x[2n+1].  Im X[0] and Im X[M] are ignored, as by FFTW's
+// halfcomplex c2r.
+template <class R, int M>
+__device__ inline void row_c2r_prep(cpx<R> (&v)[8], const cpx<R> (&x)[8], cpx<R> xm, int rt, const cpx<R>* tw,
+                                    int twsN) {
+  constexpr int T = M / 8;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int k = rt + T * m;
-    cpx<R> a = lrow[k], b = lrow[M - k];
-    if (k == 0) {
+    cpx<R> a = x[m];
+    cpx<R> b = row_partner<R, M>(x, m, rt);
+    if (m == 0 && rt == 0) {
+      b = xm;
       a.y = 0;
       b.y = 0;
     }
@@ -437,53 +510,24 @@ __device__ inline void row_c2r_prep(cpx<R> (&v)[8], cpx<R>* lrow, const cpx<R>* 
     const cpx<R> o = cmul<true>(a - bc, tw[k * twsN]);
     v[m] = e + mul_i(o);
   }
-  __syncthreads();
 }
 
-// Forward r2c of a row held in registers as packed reals z[n] (slot m at
-// index pos(m)): length-M FFT, then X[k] = (Zk + conj Z_{M-k})/2
-// - (i/2) exp(-2 pi i k/nx) (Zk - conj Z_{M-k}), k = 0..M, stored to the
-// main row and nyq.
-template <class R, int M, class POS>
-__device__ inline void row_r2c_store(cpx<R> (&v)[8], cpx<R>* lrow, cpx<R>* mainrow, cpx<R>* nyqp, int rt,
-                                     const cpx<R>* __restrict__ tw, int twsM, int twsN, const POS& pos) {
+// After the length-M forward FFT of packed reals (z: Z[rt + T*m]):
+// X[k] = (Zk + conj Z_{M-k})/2 - (i/2) exp(-2 pi i k/nx) (Zk - conj Z_{M-k}),
+// k = 0..M-1 to the main row, X[M] = Re Z0 - Im Z0 to nyq.
+template <class R, int M>
+__device__ inline void row_r2c_post(const cpx<R> (&z)[8], cpx<R>* mainrow, cpx<R>* nyqp, int rt, const cpx<R>* tw,
+                                    int twsN) {
   constexpr int T = M / 8;
-  using L = Line<R, M, 8, -1>;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) lrow[pos(m)] = v[m];
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = lrow[rt + T * m];
-  __syncthreads();
-  L::run(v, lrow, rt, tw, twsM, LinIdx{});
-#pragma unroll
-  for (int m = 0; m < 8; ++m) lrow[L::out_pos(m, rt)] = v[m];
-  __syncthreads();
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int k = rt + T * m;
-    const cpx<R> zk = lrow[k], zc = conj(lrow[(M - k) & (M - 1)]);
+    const cpx<R> zk = z[m], zc = conj(row_partner<R, M>(z, m, rt));
     const cpx<R> s = zk + zc, d = zk - zc;
-    const cpx<R> x = (R)0.5 * (s - mul_i(tw[k * twsN] * d));
-    mainrow[k] = x;
+    mainrow[k] = (R)0.5 * (s - mul_i(tw[k * twsN] * d));
   }
-  if (rt == 0) {
-    const cpx<R> z0 = lrow[0];
-    *nyqp = mk<R>(z0.x - z0.y, (R)0);
-  }
-  __syncthreads();
+  if (rt == 0) *nyqp = mk<R>(z[0].x - z[0].y, (R)0);
 }
-
-template <class R, int M>
-struct InvPos {
-  int rt;
-  __device__ inline int operator()(int m) const { return Line<R, M, 8, +1>::out_pos(m, rt); }
-};
-template <int M>
-struct EntryPos {
-  int rt;
-  __device__ inline int operator()(int m) const { return rt + (M / 8) * m; }
-};
 
 // XPASS: per row, c2r of 6 fields, u x w * scale, r2c of 3, in place into
 // fields 0..2.  Fields at W + f*fstride.
@@ -492,32 +536,54 @@ __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_convec(cpx<R>* W, in
                                                                  const cpx<R>* __restrict__ tw, int twsM, int twsN) {
   using C = RowCfg<R, M>;
   constexpr int T = C::TPR;
-  __shared__ cpx<R> lds[C::RPW * C::SR];
+  __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
+  // twiddles of the length-M FFT and of the real-FFT packing, once per
+  // workgroup (the grid is persistent: each workgroup walks row groups
+  // blockIdx.x, blockIdx.x + gridDim.x, ...)
+  __shared__ cpx<R> twM[M], twN[M];
+  for (int i = threadIdx.x; i < M; i += C::BS) {
+    twM[i] = tw[i * twsM];
+    twN[i] = tw[i * twsN];
+  }
+  __syncthreads();
   const int slot = threadIdx.x / T, rt = threadIdx.x % T;
-  const int64_t row = (int64_t)blockIdx.x * C::RPW + slot;
-  cpx<R>* lrow = lds + slot * C::SR;
+  cpx<R>* lrow = lds + slot * lds_pad(M);
   using L = Line<R, M, 8, +1>;
-  cpx<R> a[6][8];
+  using LF = Line<R, M, 8, -1>;
+  const int64_t ngroups = g.nrow / C::RPW;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row = grp * C::RPW + slot;
+    // all six rows are requested before any transform starts
+    cpx<R> raw[6][8], xm[6];
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    cpx<R>* base = W + f * fstride;
-    row_c2r_prep<R, M>(a[f], lrow, base + row * M, base + g.nmain + row, rt, tw, twsN);
-    L::run(a[f], lrow, rt, tw, twsM, LinIdx{});
-  }
-  // u x w on the packed reals: real and imaginary parts are the even and
-  // odd x points (convec_dns.f90:30-42)
-  cpx<R> p[3][8];
+    for (int f = 0; f < 6; ++f) {
+      const cpx<R>* base = W + f * fstride;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const cpx<R> ux = a[0][m], uy = a[1][m], uz = a[2][m], wx = a[3][m], wy = a[4][m], wz = a[5][m];
-    p[0][m] = mk<R>((uy.x * wz.x - uz.x * wy.x) * scale, (uy.y * wz.y - uz.y * wy.y) * scale);
-    p[1][m] = mk<R>((uz.x * wx.x - ux.x * wz.x) * scale, (uz.y * wx.y - ux.y * wz.y) * scale);
-    p[2][m] = mk<R>((ux.x * wy.x - uy.x * wx.x) * scale, (ux.y * wy.y - uy.y * wx.y) * scale);
-  }
+      for (int m = 0; m < 8; ++m) raw[f][m] = base[row * M + rt + T * m];
+      xm[f] = rt == 0 ? base[g.nmain + row] : mk<R>(0, 0);
+    }
+    cpx<R> a[6][8];
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    cpx<R>* base = W + f * fstride;
-    row_r2c_store<R, M>(p[f], lrow, base + row * M, base + g.nmain + row, rt, tw, twsM, twsN, InvPos<R, M>{rt});
+    for (int f = 0; f < 6; ++f) {
+      row_c2r_prep<R, M>(a[f], raw[f], xm[f], rt, twN, 1);
+      L::run(a[f], lrow, rt, twM, 1, LinIdx{});
+    }
+    // u x w on the packed reals: real and imaginary parts are the even and
+    // odd x points (convec_dns.f90:30-42)
+    cpx<R> p[3][8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const cpx<R> ux = a[0][m], uy = a[1][m], uz = a[2][m], wx = a[3][m], wy = a[4][m], wz = a[5][m];
+      p[0][m] = mk<R>((uy.x * wz.x - uz.x * wy.x) * scale, (uy.y * wz.y - uz.y * wy.y) * scale);
+      p[1][m] = mk<R>((uz.x * wx.x - ux.x * wz.x) * scale, (uz.y * wx.y - ux.y * wz.y) * scale);
+      p[2][m] = mk<R>((ux.x * wy.x - uy.x * wx.x) * scale, (ux.y * wy.y - uy.y * wx.y) * scale);
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cpx<R>* base = W + f * fstride;
+      LF::run(p[f], lrow, rt, twM, 1, LinIdx{});
+      row_r2c_post<R, M>(p[f], base + row * M, base + g.nmain + row, rt, twN, 1);
+    }
   }
 }
 
@@ -528,14 +594,17 @@ __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_c2r(cpx<R>* W, int64
                                                               const cpx<R>* __restrict__ tw, int twsM, int twsN) {
   using C = RowCfg<R, M>;
   constexpr int T = C::TPR;
-  __shared__ cpx<R> lds[C::RPW * C::SR];
+  __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
   const int slot = threadIdx.x / T, rt = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * C::RPW + slot;
-  cpx<R>* lrow = lds + slot * C::SR;
+  cpx<R>* lrow = lds + slot * lds_pad(M);
   cpx<R>* base = W + (int64_t)blockIdx.y * fstride;
   using L = Line<R, M, 8, +1>;
-  cpx<R> a[8];
-  row_c2r_prep<R, M>(a, lrow, base + row * M, base + g.nmain + row, rt, tw, twsN);
+  cpx<R> raw[8], a[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) raw[m] = base[row * M + rt + T * m];
+  const cpx<R> xm = rt == 0 ? base[g.nmain + row] : mk<R>(0, 0);
+  row_c2r_prep<R, M>(a, raw, xm, rt, tw, twsN);
   L::run(a, lrow, rt, tw, twsM, LinIdx{});
 #pragma unroll
   for (int m = 0; m < 8; ++m) base[row * M + L::out_pos(m, rt)] = a[m];
@@ -547,15 +616,16 @@ __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_r2c(cpx<R>* W, int64
                                                               const cpx<R>* __restrict__ tw, int twsM, int twsN) {
   using C = RowCfg<R, M>;
   constexpr int T = C::TPR;
-  __shared__ cpx<R> lds[C::RPW * C::SR];
+  __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
   const int slot = threadIdx.x / T, rt = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * C::RPW + slot;
-  cpx<R>* lrow = lds + slot * C::SR;
+  cpx<R>* lrow = lds + slot * lds_pad(M);
   cpx<R>* base = W + (int64_t)blockIdx.y * fstride;
   cpx<R> a[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) a[m] = base[row * M + rt + T * m];
-  row_r2c_store<R, M>(a, lrow, base + row * M, base + g.nmain + row, rt, tw, twsM, twsN, EntryPos<M>{rt});
+  Line<R, M, 8, -1>::run(a, lrow, rt, tw, twsM, LinIdx{});
+  row_r2c_post<R, M>(a, base + row * M, base + g.nmain + row, rt, tw, twsN);
 }
 
 // ----------------------------------------------------------------------------

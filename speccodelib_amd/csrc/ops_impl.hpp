@@ -62,6 +62,16 @@ static inline const cpx<R>* twp(const scl_plan3d& pl) {
 
 static inline dim3 grid1(int64_t n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
+template <class R, int N>
+static int32_t check_cols(const Geom& g) {
+  constexpr int CW = ColCfg<R, N>::CW;
+  if (g.mx < CW || g.mx % CW || g.ny % CW || g.nz % CW) {
+    set_error("grid (%d,%d,%d) too small for %d-column tiles", g.nx, g.ny, g.nz, CW);
+    return SCL_EINVAL;
+  }
+  return SCL_OK;
+}
+
 template <class R>
 int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int axis, int dir) {
   const Geom& g = pl.g;
@@ -69,40 +79,21 @@ int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int
   return with_line<R>(n, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
     using CC = ColCfg<R, N>;
-    constexpr int CW = (CC::CW > 8) ? CC::CW : 8;
-    if (g.mx < CW) {
-      // narrow grids: 8 columns per workgroup
-      constexpr int CW8 = 8;
-      dim3 blk(CW8 * (N / 16));
-      if (axis == 1) {
-        dim3 grd(col_grid<CW8, 1>(g), nf);
-        if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW8, 1, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
-        else hipLaunchKernelGGL((k_colfft<R, N, CW8, 1, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
-      } else {
-        dim3 grd(col_grid<CW8, 2>(g), nf);
-        if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW8, 2, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
-        else hipLaunchKernelGGL((k_colfft<R, N, CW8, 2, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
-      }
-      return check_launch("colfft8");
-    }
-    dim3 blk(CW * (N / 16));
+    SCL_CHECK((check_cols<R, N>(g)));
+    const dim3 blk(CC::BS);
+    const cpx<R>* tw = twp<R>(pl);
+    const int tws = pl.L / N;
     if (axis == 1) {
-      dim3 grd(col_grid<CW, 1>(g), nf);
-      if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW, 1, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
-      else hipLaunchKernelGGL((k_colfft<R, N, CW, 1, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+      const dim3 grd(col_grid<CC::CW, 1>(g), nf);
+      if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, 1, -1>), grd, blk, 0, pl.stream, f0, fstride, g, tw, tws);
+      else hipLaunchKernelGGL((k_colfft<R, N, 1, 1>), grd, blk, 0, pl.stream, f0, fstride, g, tw, tws);
     } else {
-      dim3 grd(col_grid<CW, 2>(g), nf);
-      if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, CW, 2, -1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
-      else hipLaunchKernelGGL((k_colfft<R, N, CW, 2, 1>), grd, blk, 0, pl.stream, f0, fstride, g, twp<R>(pl), pl.L / N);
+      const dim3 grd(col_grid<CC::CW, 2>(g), nf);
+      if (dir < 0) hipLaunchKernelGGL((k_colfft<R, N, 2, -1>), grd, blk, 0, pl.stream, f0, fstride, g, tw, tws);
+      else hipLaunchKernelGGL((k_colfft<R, N, 2, 1>), grd, blk, 0, pl.stream, f0, fstride, g, tw, tws);
     }
     return check_launch("colfft");
   });
-}
-
-// Columns per workgroup actually used for a plan (see colfft).
-template <class R, int N>
-constexpr int cw_for(int mx) {
-  return ColCfg<R, N>::CW;
 }
 
 template <class R>
@@ -110,14 +101,10 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
-    constexpr int CW = ColCfg<R, N>::CW;
-    if (g.mx < CW) {
-      hipLaunchKernelGGL((k_zinv_vort<R, N, 8>), dim3(col_grid<8, 2>(g)), dim3(8 * (N / 16)), 0, pl.stream, s,
-                         sstride, w, wstride, g, twp<R>(pl), pl.L / N);
-    } else {
-      hipLaunchKernelGGL((k_zinv_vort<R, N, CW>), dim3(col_grid<CW, 2>(g)), dim3(CW * (N / 16)), 0, pl.stream, s,
-                         sstride, w, wstride, g, twp<R>(pl), pl.L / N);
-    }
+    using CC = ColCfg<R, N>;
+    SCL_CHECK((check_cols<R, N>(g)));
+    hipLaunchKernelGGL((k_zinv_vort<R, N>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, sstride, w,
+                       wstride, g, twp<R>(pl), pl.L / N);
     return check_launch("zinv_vort");
   });
 }
@@ -128,24 +115,20 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
-    constexpr int CWF = ColCfg<R, N>::CW;
-    auto go = [&](auto CWC) -> int32_t {
-      constexpr int CW = decltype(CWC)::value;
-      dim3 grd(col_grid<CW, 2>(g)), blk(CW * (N / 16));
-      const cpx<R>* tw = twp<R>(pl);
-      const int tws = pl.L / N;
-      switch (mode) {
-        case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 0>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 1>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 2>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 3>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, CW, 4>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        default: set_error("bad mode %d", mode); return SCL_EINVAL;
-      }
-      return check_launch("zfwd_rhs");
-    };
-    if (g.mx < CWF) return go(IC<8>{});
-    return go(IC<CWF>{});
+    using CC = ColCfg<R, N>;
+    SCL_CHECK((check_cols<R, N>(g)));
+    const dim3 grd(col_grid<CC::CW, 2>(g)), blk(CC::BS);
+    const cpx<R>* tw = twp<R>(pl);
+    const int tws = pl.L / N;
+    switch (mode) {
+      case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+      case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+      case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+      case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+      case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+      default: set_error("bad mode %d", mode); return SCL_EINVAL;
+    }
+    return check_launch("zfwd_rhs");
   });
 }
 
@@ -176,7 +159,18 @@ int32_t Ops<R>::row_convec(const scl_plan3d& pl, C* W, int64_t fstride) {
   return with_half<R>(g.mx, [&](auto MC) -> int32_t {
     constexpr int M = decltype(MC)::value;
     using RC = RowCfg<R, M>;
-    hipLaunchKernelGGL((k_row_convec<R, M>), dim3((unsigned)(g.nrow / RC::RPW)), dim3(RC::BS), 0, pl.stream, W,
+    // persistent grid: as many workgroups as can be resident at once
+    static int resident = 0;
+    if (!resident) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_row_convec<R, M>, RC::BS, 0) != hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      resident = per_cu;
+    }
+    const int64_t groups = g.nrow / RC::RPW;
+    const int64_t grid = std::min<int64_t>(groups, (int64_t)resident * pl.ncu);
+    hipLaunchKernelGGL((k_row_convec<R, M>), dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W,
                        fstride, g, scale, twp<R>(pl), pl.L / M, pl.L / g.nx);
     return check_launch("row_convec");
   });

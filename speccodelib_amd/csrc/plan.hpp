@@ -9,6 +9,7 @@ struct scl_plan3d {
   int prec = SCL_SP;
   scl::Geom g{};
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   bool own_stream = false;
   void* tw = nullptr;  // twiddle table exp(-2 pi i j/L), j < L, in the plan's precision

@@ -133,7 +133,7 @@ template <> struct PlanOf<512, 8> { using L = RList<8, 8, 8>; };
 
 // One Stockham pass in registers (no data exchange).
 template <class R, int N, int E, int DIR, int NS, int RAD>
-__device__ inline void pass_compute(cpx<R> (&v)[E], int rt, const cpx<R>* __restrict__ tw, int tws) {
+__device__ inline void pass_compute(cpx<R> (&v)[E], int rt, const cpx<R>* tw, int tws) {
   constexpr int T = N / E;
   constexpr int NB = E / RAD;
 #pragma unroll
@@ -172,7 +172,7 @@ template <class R, int N, int E, int DIR, int NS, int RAD, int... REST>
 struct Passes<R, N, E, DIR, NS, RList<RAD, REST...>> {
   using Next = Passes<R, N, E, DIR, NS * RAD, RList<REST...>>;
   template <class IDX>
-  __device__ static inline void run(cpx<R> (&v)[E], cpx<R>* lds, int rt, const cpx<R>* __restrict__ tw, int tws,
+  __device__ static inline void run(cpx<R> (&v)[E], cpx<R>* lds, int rt, const cpx<R>* tw, int tws,
                                     const IDX& idx) {
     pass_compute<R, N, E, DIR, NS, RAD>(v, rt, tw, tws);
     if constexpr (sizeof...(REST) > 0) {
@@ -199,7 +199,7 @@ template <class R, int N, int E, int DIR>
 struct Line {
   using P = Passes<R, N, E, DIR, 1, typename PlanOf<N, E>::L>;
   template <class IDX>
-  __device__ static inline void run(cpx<R> (&v)[E], cpx<R>* lds, int rt, const cpx<R>* __restrict__ tw, int tws,
+  __device__ static inline void run(cpx<R> (&v)[E], cpx<R>* lds, int rt, const cpx<R>* tw, int tws,
                                     const IDX& idx) {
     P::run(v, lds, rt, tw, tws, idx);
   }
