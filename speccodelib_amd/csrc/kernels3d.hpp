@@ -109,13 +109,17 @@ struct LinIdx {
 #ifndef SCL_COL_CW
 #define SCL_COL_CW 8
 #endif
-template <class R, int N>
+template <class R, int N, int CWv = SCL_COL_CW>
 struct ColCfg {
-  static constexpr int CW = SCL_COL_CW;
+  static constexpr int CW = CWv;
   static constexpr int E = N <= 512 ? 8 : 16;
   static constexpr int T = N / E;
   static constexpr int BS = CW * T;
 };
+// The fused z-pass kernels (vorticity in, RK4 epilogue out) take CW as a
+// template argument: 16 columns (whole 128-byte lines) where the grid
+// allows, since z-rows are nx*ny/2 entries apart and half-line segments
+// cost more there than along y.
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD
 // b mod 8).  Map block b to logical tile (b mod 8) * (G/8) + b/8 so that each
@@ -149,11 +153,11 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
 
 // ZINV: z-inverse of u and of w = i k x u.  s: stage state (3 fields at
 // stride sstride); out: 6 fields (u then w) at stride wstride.
-template <class R, int N>
-__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_zinv_vort(const cpx<R>* __restrict__ s, int64_t sstride,
+template <class R, int N, int CWZ>
+__global__ __launch_bounds__((ColCfg<R, N, CWZ>::BS)) void k_zinv_vort(const cpx<R>* __restrict__ s, int64_t sstride,
                                                                  cpx<R>* __restrict__ w, int64_t wstride, Geom g,
                                                                  const cpx<R>* __restrict__ tw, int tws) {
-  using CC = ColCfg<R, N>;
+  using CC = ColCfg<R, N, CWZ>;
   constexpr int CW = CC::CW, E = CC::E, T = CC::T;
   __shared__ cpx<R> lds[lds_pad(N) * CW];
   __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
@@ -227,12 +231,12 @@ struct RhsParams {
 //   MODE 3 (:198-213):  k = dt N / E; du += 2k; S = E E (v + k)
 //   MODE 4 (:215-237):  k = dt N / E / E; du += k; v = E E (v + du/6); v = 0 where k2 >= kc2
 // with E = exp(-nu k2 dt / 2) (:90).
-template <class R, int N, int MODE>
-__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V, cpx<R>* DU,
+template <class R, int N, int MODE, int CWZ>
+__global__ __launch_bounds__((ColCfg<R, N, CWZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V, cpx<R>* DU,
                                                                 cpx<R>* S, const cpx<R>* st,
                                                                 const double* __restrict__ coef, RhsParams<R> p, Geom g,
                                                                 const cpx<R>* __restrict__ tw, int tws) {
-  using CC = ColCfg<R, N>;
+  using CC = ColCfg<R, N, CWZ>;
   constexpr int CW = CC::CW, E = CC::E, T = CC::T;
   __shared__ cpx<R> lds[lds_pad(N) * CW];
   __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier

@@ -62,9 +62,8 @@ static inline const cpx<R>* twp(const scl_plan3d& pl) {
 
 static inline dim3 grid1(int64_t n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
-template <class R, int N>
+template <class R, int N, int CW = ColCfg<R, N>::CW>
 static int32_t check_cols(const Geom& g) {
-  constexpr int CW = ColCfg<R, N>::CW;
   if (g.mx < CW || g.mx % CW || g.ny % CW || g.nz % CW) {
     set_error("grid (%d,%d,%d) too small for %d-column tiles", g.nx, g.ny, g.nz, CW);
     return SCL_EINVAL;
@@ -96,16 +95,29 @@ int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int
   });
 }
 
+// Column count of the fused z-pass kernels for a grid.
+template <class R>
+static bool zcol16(const Geom& g) {
+  return sizeof(R) == 4 && g.mx % 16 == 0 && g.ny % 16 == 0;
+}
+
 template <class R>
 int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* w, int64_t wstride) {
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
-    using CC = ColCfg<R, N>;
-    SCL_CHECK((check_cols<R, N>(g)));
-    hipLaunchKernelGGL((k_zinv_vort<R, N>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, sstride, w,
-                       wstride, g, twp<R>(pl), pl.L / N);
-    return check_launch("zinv_vort");
+    auto go = [&](auto CWC) -> int32_t {
+      constexpr int CW = decltype(CWC)::value;
+      using CC = ColCfg<R, N, CW>;
+      SCL_CHECK((check_cols<R, N, CW>(g)));
+      hipLaunchKernelGGL((k_zinv_vort<R, N, CW>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, sstride, w,
+                         wstride, g, twp<R>(pl), pl.L / N);
+      return check_launch("zinv_vort");
+    };
+    if constexpr (sizeof(R) == 4) {
+      if (zcol16<R>(g)) return go(IC<16>{});
+    }
+    return go(IC<8>{});
   });
 }
 
@@ -115,20 +127,27 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
-    using CC = ColCfg<R, N>;
-    SCL_CHECK((check_cols<R, N>(g)));
-    const dim3 grd(col_grid<CC::CW, 2>(g)), blk(CC::BS);
-    const cpx<R>* tw = twp<R>(pl);
-    const int tws = pl.L / N;
-    switch (mode) {
-      case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-      case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-      case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-      case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-      case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-      default: set_error("bad mode %d", mode); return SCL_EINVAL;
+    auto go = [&](auto CWC) -> int32_t {
+      constexpr int CW = decltype(CWC)::value;
+      using CC = ColCfg<R, N, CW>;
+      SCL_CHECK((check_cols<R, N, CW>(g)));
+      const dim3 grd(col_grid<CW, 2>(g)), blk(CC::BS);
+      const cpx<R>* tw = twp<R>(pl);
+      const int tws = pl.L / N;
+      switch (mode) {
+        case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        default: set_error("bad mode %d", mode); return SCL_EINVAL;
+      }
+      return check_launch("zfwd_rhs");
+    };
+    if constexpr (sizeof(R) == 4) {
+      if (zcol16<R>(g)) return go(IC<16>{});
     }
-    return check_launch("zfwd_rhs");
+    return go(IC<8>{});
   });
 }
 
