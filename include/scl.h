@@ -70,6 +70,12 @@ int32_t scl_destroyplan3d(void);
  * c2r uses exp(+ik.x), r2c exp(-ik.x); c2r(r2c(a)) = nx*ny*nz*a. */
 int32_t scl_c2r3d(scl_plan3d *plan, void *a);
 int32_t scl_r2c3d(scl_plan3d *plan, void *a);
+/* The same transforms under FFTW 2.x's Fortran-77 names, for callers left
+ * unmodified (convec_dns.f90:22-27,45-47 call them by reference): the global
+ * plan's grid, plan token and third argument unused, abort() on failure
+ * (the reference has no error path there). */
+void rfftwnd_f77_one_complex_to_real_(int64_t *plan, void *a, void *ignore);
+void rfftwnd_f77_one_real_to_complex_(int64_t *plan, void *a, void *ignore);
 
 /* ---------------------------------------------------------------------
  * RHS blocks on host arrays (reference signatures; plan = NULL uses the

@@ -884,3 +884,23 @@ extern "C" int32_t scl_dns_diagnostics(scl_dns* d, scl_diag* out) {
     return divmax_run<R>(p, (const cpx<R>*)d->fields, &out->divmax);
   });
 }
+
+// ---------------------------------------------------------------------------
+// FFTW 2.x Fortran-77 execution entry points (convec_dns.f90:22-27,45-47),
+// by the names unmodified reference code calls: in place on the global
+// plan's grid; the plan token and the `ignore` argument are not used.
+extern "C" void rfftwnd_f77_one_complex_to_real_(int64_t* /*plan*/, void* a, void* /*ignore*/) {
+  const int32_t st = scl_c2r3d(nullptr, a);
+  if (st != SCL_OK) {
+    fprintf(stderr, "libscl: rfftwnd_f77_one_complex_to_real failed: %s\n", scl_last_error());
+    abort();
+  }
+}
+
+extern "C" void rfftwnd_f77_one_real_to_complex_(int64_t* /*plan*/, void* a, void* /*ignore*/) {
+  const int32_t st = scl_r2c3d(nullptr, a);
+  if (st != SCL_OK) {
+    fprintf(stderr, "libscl: rfftwnd_f77_one_real_to_complex failed: %s\n", scl_last_error());
+    abort();
+  }
+}
