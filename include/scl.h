@@ -116,6 +116,29 @@ int32_t scl_projection(scl_plan3d *plan, void *vx, void *vy, void *vz, const voi
 int32_t scl_ratesdudt(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, void *nx_, void *ny_,
                       void *nz_, int32_t forced, double fkmax, double eps, double eta);
 
+/* smag(vx,vy,vz,t11,t12,t13,t22,t23,t33,kx,ky,kz,lx1,ly,lz,nx,ny,nz,delta)
+ * (smag.f90:1-78): tau_ij = FFT(-2 c1 delta^2 |S| S_ij)/N^3, c1 = 0.026. */
+int32_t scl_smag(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, void *t11, void *t12, void *t13,
+                 void *t22, void *t23, void *t33, const void *kx, const void *ky, const void *kz, int32_t lx1,
+                 int32_t ly, int32_t lz, int32_t nx, int32_t ny, int32_t nz, double delta);
+
+/* dynsmag(ux,uy,uz,t11,...,t33,kx,ky,kz,g_test,lx1,ly,lz,nx,ny,nz,lx,delta,delta_test,mm,lm,cs2,update)
+ * (dynsmag.f90:1-236).  kx, ky, kz, g_test are 3D real arrays (lx1,ly,lz);
+ * kx/ky/kz must be wavenumber()'s, g_test is used as given.  update != 0:
+ * Germano-Lilly cs2 = lm/mm (mm, lm, cs2 returned); update == 0: *cs2 is
+ * read (the static branch, :196-233). */
+int32_t scl_dynsmag(scl_plan3d *plan, const void *ux, const void *uy, const void *uz, void *t11, void *t12, void *t13,
+                    void *t22, void *t23, void *t33, const void *kx, const void *ky, const void *kz,
+                    const void *g_test, int32_t lx1, int32_t ly, int32_t lz, int32_t nx, int32_t ny, int32_t nz,
+                    int32_t lx, double delta, double delta_test, double *mm, double *lm, double *cs2, int32_t update);
+
+/* convecphi_dns(vx,vy,vz,phi,wphi,kx,ky,kz,lx1,ly,lz,nx,ny,nz)
+ * (convecphi_dns.f90:1-49): wphi = i k . FFT(u phi)/N^3, the reference's
+ * sign (+div(u phi)). */
+int32_t scl_convecphi_dns(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, const void *phi,
+                          void *wphi, const void *kx, const void *ky, const void *kz, int32_t lx1, int32_t ly,
+                          int32_t lz, int32_t nx, int32_t ny, int32_t nz);
+
 /* initialize(vx,vy,vz,kx,ky,kz,k2,iseed,lx1,ly,lz,akp,u0,kcut)
  * (initialize.f90:373-463 draw order, :328-332 complex projection), host
  * only, in the real type of prec.  kcut <= 0 selects 2*lx/3. */
@@ -136,7 +159,18 @@ typedef struct scl_dns_params {
   int32_t forced;      /* 0 none, 1 constant power (force.f90), 2 helical (force_heli.f90) */
   double fkmax, eps, eta;
   int32_t dealias_rhs; /* 1: also zero k2 >= kcut^2 in every RHS (default 0: once per step) */
-  int32_t use_graph;   /* 1: replay one captured step as a hipGraph */
+  int32_t use_graph;   /* 1: replay one captured step as a hipGraph (not with sgs = 2) */
+  /* LES (dynsmag.f90, smag.f90; RHS as spectral_les.f90:205-238) */
+  int32_t sgs;         /* 0 none, 1 static Smagorinsky, 2 dynamic (Germano-Lilly) */
+  double cs2;          /* static C_s^2 (smag.f90's c1 = 0.026); dynamic: start value */
+  double delta;        /* filter width; <= 0 selects 2 pi / nx */
+  double delta_test;   /* dynamic: test-filter width; <= 0 selects 2 delta */
+  double k_test;       /* dynamic: test filter passes k2 <= k_test^2; <= 0 selects (2 nx/6)/2 */
+  int32_t iupdate;     /* dynamic: new cs2 at the first stage of every iupdate-th step */
+  /* passive scalar (convecphi_dns.f90, spectral_dns_scalar.f90:236-272) */
+  int32_t scalar;      /* 1: advance phi with -div(u phi) - grad * u_z, diffusivity nu/schmidt */
+  double grad;         /* mean scalar gradient along z */
+  double schmidt;      /* Sc (Prandtl number pr of the reference) */
 } scl_dns_params;
 
 typedef struct scl_diag {
@@ -147,11 +181,17 @@ typedef struct scl_diag {
   double divmax;       /* max over the grid of |div u| (physical space) */
   double divrms_spec;  /* sqrt(sum' |k.v|^2) */
   int32_t nonfinite;   /* 1 if any NaN/Inf in v */
+  double scalar_variance;    /* sum' |phi|^2 (= <phi^2>/2) */
+  double scalar_dissipation; /* 2 (nu/Sc) sum' k^2 |phi|^2 */
+  double cs2;                /* current Smagorinsky coefficient (sgs > 0) */
 } scl_diag;
 
 int32_t scl_dns_create(scl_plan3d *plan, const scl_dns_params *p, scl_dns **out);
 int32_t scl_dns_destroy(scl_dns *d);
 int32_t scl_dns_set_velocity(scl_dns *d, const void *vx, const void *vy, const void *vz);
+/* passive scalar (scalar = 1), host array in the reference layout */
+int32_t scl_dns_set_scalar(scl_dns *d, const void *phi);
+int32_t scl_dns_get_scalar(scl_dns *d, void *phi);
 int32_t scl_dns_get_velocity(scl_dns *d, void *vx, void *vy, void *vz);
 int32_t scl_dns_initialize(scl_dns *d, int32_t iseed, double akp, double u0);
 /* nsteps IF-RK4 steps, asynchronous on the plan's stream */

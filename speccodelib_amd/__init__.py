@@ -45,6 +45,15 @@ class ScDnsParams(ctypes.Structure):
         ("eta", ctypes.c_double),
         ("dealias_rhs", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("sgs", ctypes.c_int32),
+        ("cs2", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("delta_test", ctypes.c_double),
+        ("k_test", ctypes.c_double),
+        ("iupdate", ctypes.c_int32),
+        ("scalar", ctypes.c_int32),
+        ("grad", ctypes.c_double),
+        ("schmidt", ctypes.c_double),
     ]
 
 
@@ -57,6 +66,9 @@ class ScDiag(ctypes.Structure):
         ("divmax", ctypes.c_double),
         ("divrms_spec", ctypes.c_double),
         ("nonfinite", ctypes.c_int32),
+        ("scalar_variance", ctypes.c_double),
+        ("scalar_dissipation", ctypes.c_double),
+        ("cs2", ctypes.c_double),
     ]
 
 
@@ -108,6 +120,11 @@ def lib():
             "scl_dns_helicity_spectrum": ([P, P, I32], I32),
             "scl_dns_diagnostics": ([P, ctypes.POINTER(ScDiag)], I32),
             "scl_dns_step_timed": ([P, I32, P, P, P], I32),
+            "scl_dns_set_scalar": ([P, P], I32),
+            "scl_dns_get_scalar": ([P, P], I32),
+            "scl_smag": ([P] + [P] * 12 + [I32] * 6 + [D], I32),
+            "scl_dynsmag": ([P] + [P] * 13 + [I32] * 7 + [D, D, P, P, P, I32], I32),
+            "scl_convecphi_dns": ([P] + [P] * 8 + [I32] * 6, I32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -220,6 +237,29 @@ class Plan3D:
     def rk4step(self, vx, vy, vz, nu, dt, forced=1, fkmax=2.0, eps=0.1, eta=0.0):
         _check(lib().scl_rk4step(self.h, *map(_ptr, (vx, vy, vz)), nu, dt, forced, fkmax, eps, eta))
 
+    def smag(self, vx, vy, vz, kx, ky, kz, delta):
+        """smag.f90:1-78: returns (t11, t12, t13, t22, t23, t33)."""
+        t = [self.zeros() for _ in range(6)]
+        _check(lib().scl_smag(self.h, *map(_ptr, (vx, vy, vz, *t, kx, ky, kz)), *self._dims(), self.nx, self.ny, self.nz, delta))
+        return tuple(t)
+
+    def dynsmag(self, ux, uy, uz, kx3, ky3, kz3, g_test, delta, delta_test, cs2=0.0, update=True):
+        """dynsmag.f90:1-236: returns (tau (6), mm, lm, cs2).  kx3/ky3/kz3 and
+        g_test are real 3D arrays (lz, ly, lx1)."""
+        t = [self.zeros() for _ in range(6)]
+        mm, lm, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(cs2)
+        _check(lib().scl_dynsmag(self.h, *map(_ptr, (ux, uy, uz, *t, kx3, ky3, kz3, g_test)), *self._dims(),
+                                 self.nx, self.ny, self.nz, self.nx // 2, delta, delta_test,
+                                 ctypes.byref(mm), ctypes.byref(lm), ctypes.byref(c), int(update)))
+        return tuple(t), mm.value, lm.value, c.value
+
+    def convecphi_dns(self, vx, vy, vz, phi, kx, ky, kz):
+        """convecphi_dns.f90:1-49: wphi = i k . FFT(u phi)/N^3 (the reference's sign)."""
+        w = self.zeros()
+        _check(lib().scl_convecphi_dns(self.h, *map(_ptr, (vx, vy, vz, phi, w, kx, ky, kz)), *self._dims(),
+                                       self.nx, self.ny, self.nz))
+        return w
+
     def wavenumber(self):
         return wavenumber(self.nx, self.ny, self.nz, self.prec)
 
@@ -253,9 +293,12 @@ def initialize(nx, ny=None, nz=None, iseed=1, akp=4.0, u0=1.0, prec=SP, kcut=-1.
 class DNS:
     """Device-resident IF-RK4 solver: the dns-deform.f90 loop body."""
 
-    def __init__(self, plan: Plan3D, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, eta=0.0, dealias_rhs=False, use_graph=True):
+    def __init__(self, plan: Plan3D, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, eta=0.0, dealias_rhs=False,
+                 use_graph=True, sgs=0, cs2=0.0, delta=0.0, delta_test=0.0, k_test=0.0, iupdate=10, scalar=False,
+                 grad=0.0, schmidt=1.0):
         self.plan = plan
-        self.params = ScDnsParams(nu, dt, int(forced), fkmax, eps, eta, int(dealias_rhs), int(use_graph))
+        self.params = ScDnsParams(nu, dt, int(forced), fkmax, eps, eta, int(dealias_rhs), int(use_graph), int(sgs), cs2,
+                                  delta, delta_test, k_test, int(iupdate), int(scalar), grad, schmidt)
         h = ctypes.c_void_p()
         _check(lib().scl_dns_create(plan.h, ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
@@ -276,6 +319,15 @@ class DNS:
         out = [self.plan.zeros() for _ in range(3)]
         _check(lib().scl_dns_get_velocity(self.h, *map(_ptr, out)))
         return tuple(out)
+
+    def set_scalar(self, phi):
+        _field(phi, self.plan.prec, self.plan.shape)
+        _check(lib().scl_dns_set_scalar(self.h, _ptr(phi)))
+
+    def get_scalar(self):
+        out = self.plan.zeros()
+        _check(lib().scl_dns_get_scalar(self.h, _ptr(out)))
+        return out
 
     def initialize(self, iseed=1, akp=4.0, u0=1.0):
         _check(lib().scl_dns_initialize(self.h, iseed, akp, u0))

@@ -11,6 +11,7 @@
 #include "diag.hpp"
 #include "ic.hpp"
 #include "ops.hpp"
+#include "ops_ext.hpp"
 
 namespace scl {
 
@@ -508,13 +509,26 @@ static RhsParams<R> rhs_params(const scl_plan3d* p, double nu, double dt, int fo
   return rp;
 }
 
+// Solver fields: V(nc) DU(nc) S(nc) W(...), nc = 3 velocity components
+// (+1 scalar), each fs entries; coef: forcing {a, b, E_f, Omega_f, uw} and
+// Smagorinsky {cs2, mm, lm} (8 doubles on the device).
 struct DevFields {
   void* base;
   int64_t fs;
+  int nc = 3;
+  double* coef = nullptr;
   template <class R>
   cpx<R>* f(int i) const {
     return reinterpret_cast<cpx<R>*>(base) + i * fs;
   }
+  template <class R>
+  cpx<R>* V() const { return f<R>(0); }
+  template <class R>
+  cpx<R>* DU() const { return f<R>(nc); }
+  template <class R>
+  cpx<R>* S() const { return f<R>(2 * nc); }
+  template <class R>
+  cpx<R>* W() const { return f<R>(3 * nc); }
 };
 
 // Event hooks for per-kernel timing (nullptr: none).
@@ -532,21 +546,21 @@ struct Timer {
   }
 };
 
-// RHS of stage `mode` (0: N(st) into W[0..2]) with the fields of a solver:
-// V = f(0..2), DU = f(3..5), S = f(6..8), W = f(9..14).
+// RHS of stage `mode` (0: N(st) into W[0..2]) with the fields of a solver
+// (the fused five-pass DNS RHS; nc = 3).
 template <class R>
 static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsParams<R>& rp, double eps, double eta,
                         double fkmax, const cpx<R>* st, Timer* tm) {
   const int64_t fs = d.fs;
-  cpx<R>* V = d.f<R>(0);
-  cpx<R>* DU = d.f<R>(3);
-  cpx<R>* S = d.f<R>(6);
-  cpx<R>* W = d.f<R>(9);
+  cpx<R>* V = d.V<R>();
+  cpx<R>* DU = d.DU<R>();
+  cpx<R>* S = d.S<R>();
+  cpx<R>* W = d.W<R>();
   Timer none;
   if (!tm) tm = &none;
   if (rp.forced) {
     tm->mark(SCL_K_FORCE);
-    SCL_CHECK(Ops<R>::force_coef(*p, st, fs, fkmax, eps, eta, rp.forced, p->coef));
+    SCL_CHECK(Ops<R>::force_coef(*p, st, fs, fkmax, eps, eta, rp.forced, d.coef));
   }
   tm->mark(SCL_K_ZINV);
   SCL_CHECK(Ops<R>::zinv_vort(*p, st, fs, W, fs));
@@ -557,7 +571,7 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   tm->mark(SCL_K_YFWD);
   SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
   tm->mark(SCL_K_ZFWD);
-  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, p->coef, rp));
+  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp));
   tm->mark(SCL_K_HERM);
   FieldPtrs<R> fp{};
   if (mode == 0) {
@@ -582,9 +596,51 @@ template <class R>
 static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm) {
   const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
   for (int stage = 1; stage <= 4; ++stage) {
-    const cpx<R>* st = stage == 1 ? d.f<R>(0) : d.f<R>(6);
+    const cpx<R>* st = stage == 1 ? d.V<R>() : d.S<R>();
     SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm));
   }
+  return SCL_OK;
+}
+
+// The LES and/or scalar RHS (multi-pass) for stage `mode`.
+template <class R>
+static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st,
+                            Timer* tm) {
+  const int64_t fs = d.fs;
+  Timer none;
+  if (!tm) tm = &none;
+  ExtLayout L{prm.sgs ? 1 : 0, prm.scalar ? 1 : 0};
+  ExtParams<R> ep;
+  ep.r = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  ep.nu_phi = (R)(prm.nu / (prm.schmidt > 0 ? prm.schmidt : 1.0));
+  ep.grad = (R)prm.grad;
+  ep.sign_phi = 1;
+  const double delta = prm.delta > 0 ? prm.delta : 2.0 * M_PI / p->g.nx;
+  if (prm.forced) {
+    tm->mark(SCL_K_FORCE);
+    SCL_CHECK(Ops<R>::force_coef(*p, st, fs, prm.fkmax, prm.eps, prm.eta, prm.forced, d.coef));
+  }
+  tm->mark(SCL_K_ZFWD);
+  SCL_CHECK(ExtOps<R>::rhs(*p, mode, L, d.W<R>(), fs, d.V<R>(), d.DU<R>(), d.S<R>(), st, d.coef, ep,
+                           (R)(delta * delta)));
+  tm->mark(SCL_K_HERM);
+  FieldPtrs<R> fp{};
+  const int nc = d.nc;
+  if (mode == 0) {
+    for (int q = 0; q < nc; ++q) fp.p[q] = d.W<R>() + q * fs;
+    fp.n = nc;
+  } else if (mode < 4) {
+    for (int q = 0; q < nc; ++q) {
+      fp.p[q] = d.DU<R>() + q * fs;
+      fp.p[nc + q] = d.S<R>() + q * fs;
+    }
+    fp.n = 2 * nc;
+  } else {
+    for (int q = 0; q < nc; ++q) fp.p[q] = d.V<R>() + q * fs;
+    fp.n = nc;
+  }
+  SCL_CHECK(Ops<R>::herm_fix(*p, fp));
+  tm->mark(-1);
   return SCL_OK;
 }
 
@@ -596,7 +652,7 @@ extern "C" int32_t scl_ratesdudt(scl_plan3d* p, const void* vx, const void* vy, 
     using R = decltype(zero);
     void* s;
     SCL_CHECK(plan_scratch(p, 15, &s));
-    DevFields d{s, p->fstride()};
+    DevFields d{s, p->fstride(), 3, p->coef};
     const void* vi[3] = {vx, vy, vz};
     for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, vi[f], d.f<R>(6 + f)));
     const RhsParams<R> rp = rhs_params<R>(p, 0.0, 0.0, forced, fkmax, 0);
@@ -615,7 +671,7 @@ extern "C" int32_t scl_rk4step(scl_plan3d* p, void* vx, void* vy, void* vz, doub
     using R = decltype(zero);
     void* s;
     SCL_CHECK(plan_scratch(p, 15, &s));
-    DevFields d{s, p->fstride()};
+    DevFields d{s, p->fstride(), 3, p->coef};
     void* vi[3] = {vx, vy, vz};
     for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, vi[f], d.f<R>(f)));
     scl_dns_params prm{};
@@ -640,69 +696,124 @@ extern "C" int32_t scl_initialize3d(int32_t prec, void* vx, void* vy, void* vz, 
 
 // ---------------------------------------------------------------------------
 // Device-resident solver
+static DevFields dev_fields(const scl_dns* d) {
+  DevFields f{d->fields, d->plan->fstride(), d->nc, d->coef};
+  return f;
+}
+
+static void free_dns(scl_dns* d) {
+  if (d->exec) hipGraphExecDestroy(d->exec);
+  if (d->graph) hipGraphDestroy(d->graph);
+  if (d->fields) hipFree(d->fields);
+  if (d->dyn) hipFree(d->dyn);
+  if (d->gt) hipFree(d->gt);
+  if (d->part) hipFree(d->part);
+  if (d->coef) hipFree(d->coef);
+  delete d;
+}
+
 extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_dns** out) {
   if (!out || !prm) return set_error("null argument"), SCL_EINVAL;
   *out = nullptr;
   SCL_PLAN(p);
   if (prm->forced < 0 || prm->forced > 2) return set_error("forced must be 0, 1 or 2"), SCL_EINVAL;
+  if (prm->sgs < 0 || prm->sgs > 2) return set_error("sgs must be 0, 1 or 2"), SCL_EINVAL;
+  if (prm->scalar && !(prm->schmidt > 0)) return set_error("scalar needs schmidt > 0"), SCL_EINVAL;
   auto* d = new scl_dns();
   d->plan = p;
   d->p = *prm;
-  const size_t bytes = 15 * (size_t)p->fstride() * p->esize();
-  if (hipMalloc(&d->fields, bytes) != hipSuccess) {
+  d->ext = prm->sgs != 0 || prm->scalar != 0;
+  d->nc = prm->scalar ? 4 : 3;
+  d->nw = d->ext ? ExtLayout{prm->sgs ? 1 : 0, prm->scalar ? 1 : 0}.nin() : 6;
+  if (d->ext) {
+    const ExtLayout L{prm->sgs ? 1 : 0, prm->scalar ? 1 : 0};
+    d->nw = std::max(L.nin(), L.nout());
+  }
+  if (prm->sgs == 2) d->p.use_graph = 0;  // the update step differs from the others
+  const size_t fb = (size_t)p->fstride() * p->esize();
+  const size_t bytes = (size_t)(3 * d->nc + d->nw) * fb;
+  auto fail = [&](size_t b) {
     hipGetLastError();
-    delete d;
-    return set_error("device allocation of %zu bytes for the solver failed", bytes), SCL_ENOMEM;
+    free_dns(d);
+    set_error("device allocation of %zu bytes for the solver failed", b);
+    return SCL_ENOMEM;
+  };
+  if (hipMalloc(&d->fields, bytes) != hipSuccess) return fail(bytes);
+  if (hipMalloc((void**)&d->coef, 8 * sizeof(double)) != hipSuccess) return fail(64);
+  if (prm->sgs == 2) {
+    if (hipMalloc(&d->dyn, 24 * fb) != hipSuccess) return fail(24 * fb);
+    if (hipMalloc(&d->gt, (size_t)p->fstride() * (p->esize() / 2)) != hipSuccess) return fail(fb / 2);
+    if (hipMalloc((void**)&d->part, (2 * ExtOps<float>::DIAG_BLOCKS_DYN + 8) * sizeof(double)) != hipSuccess)
+      return fail(8192);
   }
   hipMemsetAsync(d->fields, 0, bytes, p->stream);
-  hipStreamSynchronize(p->stream);
+  double c0[8] = {0, 0, 0, 0, 0, prm->cs2, 0, 0};
+  hipMemcpyAsync(d->coef, c0, sizeof(c0), hipMemcpyHostToDevice, p->stream);
+  if (prm->sgs == 2) {
+    const double kc = 2.0 * (p->g.nx / 2) / 3.0;
+    const double kt = prm->k_test > 0 ? prm->k_test : kc / 2.0;
+    int32_t st = by_prec(p->prec, [&](auto zero) -> int32_t {
+      using R = decltype(zero);
+      return ExtOps<R>::gtest(*p, (R*)d->gt, (R)kt);
+    });
+    if (st != SCL_OK) {
+      free_dns(d);
+      return st;
+    }
+  }
+  if (hipStreamSynchronize(p->stream) != hipSuccess) {
+    free_dns(d);
+    return set_error("solver initialisation failed"), SCL_EHIP;
+  }
   *out = d;
   return SCL_OK;
-}
-
-static void drop_graph(scl_dns* d) {
-  if (d->exec) hipGraphExecDestroy(d->exec);
-  if (d->graph) hipGraphDestroy(d->graph);
-  d->exec = nullptr;
-  d->graph = nullptr;
 }
 
 extern "C" int32_t scl_dns_destroy(scl_dns* d) {
   if (!d) return SCL_OK;
   hipStreamSynchronize(d->plan->stream);
-  drop_graph(d);
-  if (d->fields) hipFree(d->fields);
-  delete d;
+  free_dns(d);
   return SCL_OK;
 }
 
-extern "C" int32_t scl_dns_set_velocity(scl_dns* d, const void* vx, const void* vy, const void* vz) {
-  if (!d) return set_error("null solver"), SCL_EINVAL;
+static int32_t dns_io(scl_dns* d, int first, int n, const void* const* in, void* const* out) {
   scl_plan3d* p = d->plan;
   return by_prec(p->prec, [&](auto zero) -> int32_t {
     using R = decltype(zero);
     void* s;
     SCL_CHECK(plan_scratch(p, 0, &s));
-    DevFields f{d->fields, p->fstride()};
-    const void* vi[3] = {vx, vy, vz};
-    for (int q = 0; q < 3; ++q) SCL_CHECK(upload<R>(p, vi[q], f.f<R>(q)));
+    DevFields f = dev_fields(d);
+    for (int q = 0; q < n; ++q) {
+      if (in) SCL_CHECK(upload<R>(p, in[q], f.f<R>(first + q)));
+      else SCL_CHECK(download<R>(p, f.f<R>(first + q), out[q]));
+    }
     SCL_HIP(hipStreamSynchronize(p->stream));
     return SCL_OK;
   });
 }
 
+extern "C" int32_t scl_dns_set_velocity(scl_dns* d, const void* vx, const void* vy, const void* vz) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  const void* vi[3] = {vx, vy, vz};
+  return dns_io(d, 0, 3, vi, nullptr);
+}
+
 extern "C" int32_t scl_dns_get_velocity(scl_dns* d, void* vx, void* vy, void* vz) {
   if (!d) return set_error("null solver"), SCL_EINVAL;
-  scl_plan3d* p = d->plan;
-  return by_prec(p->prec, [&](auto zero) -> int32_t {
-    using R = decltype(zero);
-    void* s;
-    SCL_CHECK(plan_scratch(p, 0, &s));
-    DevFields f{d->fields, p->fstride()};
-    void* vo[3] = {vx, vy, vz};
-    for (int q = 0; q < 3; ++q) SCL_CHECK(download<R>(p, f.f<R>(q), vo[q]));
-    return SCL_OK;
-  });
+  void* vo[3] = {vx, vy, vz};
+  return dns_io(d, 0, 3, nullptr, vo);
+}
+
+extern "C" int32_t scl_dns_set_scalar(scl_dns* d, const void* phi) {
+  if (!d || d->nc < 4) return set_error("solver has no scalar"), SCL_EINVAL;
+  const void* vi[1] = {phi};
+  return dns_io(d, 3, 1, vi, nullptr);
+}
+
+extern "C" int32_t scl_dns_get_scalar(scl_dns* d, void* phi) {
+  if (!d || d->nc < 4) return set_error("solver has no scalar"), SCL_EINVAL;
+  void* vo[1] = {phi};
+  return dns_io(d, 3, 1, nullptr, vo);
 }
 
 extern "C" int32_t scl_dns_initialize(scl_dns* d, int32_t iseed, double akp, double u0) {
@@ -720,13 +831,33 @@ extern "C" int32_t scl_dns_initialize(scl_dns* d, int32_t iseed, double akp, dou
 }
 
 template <class R>
+static int32_t dns_step_once(scl_dns* d, Timer* tm) {
+  scl_plan3d* p = d->plan;
+  DevFields f = dev_fields(d);
+  if (!d->ext) return rk4_step_dev<R>(p, f, d->p, tm);
+  const scl_dns_params& prm = d->p;
+  if (prm.sgs == 2 && d->steps % (prm.iupdate > 0 ? prm.iupdate : 1) == 0) {
+    const double delta = prm.delta > 0 ? prm.delta : 2.0 * M_PI / p->g.nx;
+    const double dtest = prm.delta_test > 0 ? prm.delta_test : 2.0 * delta;
+    SCL_CHECK(ExtOps<R>::dyn_update(*p, f.V<R>(), (cpx<R>*)d->dyn, f.fs, (const R*)d->gt, (R)delta, (R)dtest, d->coef,
+                                    d->part));
+  }
+  for (int stage = 1; stage <= 4; ++stage) {
+    const cpx<R>* st = stage == 1 ? f.V<R>() : f.S<R>();
+    SCL_CHECK(rk_stage_ext<R>(p, f, stage, prm, st, tm));
+  }
+  return SCL_OK;
+}
+
+template <class R>
 static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
   scl_plan3d* p = d->plan;
-  DevFields f{d->fields, p->fstride()};
   if (d->p.use_graph && !tm) {
     if (!d->exec) {
       SCL_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-      int32_t st = rk4_step_dev<R>(p, f, d->p, nullptr);
+      const int64_t keep = d->steps;
+      int32_t st = dns_step_once<R>(d, nullptr);
+      d->steps = keep;
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(p->stream, &gr);
       if (st != SCL_OK) {
@@ -737,11 +868,16 @@ static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
       d->graph = gr;
       SCL_HIP(hipGraphInstantiate(&d->exec, d->graph, nullptr, nullptr, 0));
     }
-    for (int i = 0; i < nsteps; ++i) SCL_HIP(hipGraphLaunch(d->exec, p->stream));
+    for (int i = 0; i < nsteps; ++i) {
+      SCL_HIP(hipGraphLaunch(d->exec, p->stream));
+      d->steps += 1;
+    }
   } else {
-    for (int i = 0; i < nsteps; ++i) SCL_CHECK(rk4_step_dev<R>(p, f, d->p, tm));
+    for (int i = 0; i < nsteps; ++i) {
+      SCL_CHECK(dns_step_once<R>(d, tm));
+      d->steps += 1;
+    }
   }
-  d->steps += nsteps;
   d->time += nsteps * d->p.dt;
   return SCL_OK;
 }
@@ -843,6 +979,24 @@ static int32_t divmax_run(scl_plan3d* p, const cpx<R>* v, double* out) {
   return SCL_OK;
 }
 
+template <class R>
+static int32_t scalar_diag_run(scl_plan3d* p, const cpx<R>* phi, double* var, double* kvar) {
+  void* buf;
+  SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * (2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &buf));
+  double* part = (double*)buf;
+  double* sums = part + 2 * DIAG_BLOCKS;
+  hipLaunchKernelGGL((k_scalar_diag<R>), dim3(DIAG_BLOCKS), dim3(256), 0, p->stream, phi, p->g, part);
+  SCL_CHECK(check_launch("scalar_diag"));
+  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(2), 0, p->stream, part, DIAG_BLOCKS, sums);
+  SCL_CHECK(check_launch("scalar_sum"));
+  double h[2];
+  SCL_HIP(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+  SCL_HIP(hipStreamSynchronize(p->stream));
+  *var = h[0];
+  *kvar = h[1];
+  return SCL_OK;
+}
+
 extern "C" int32_t scl_dns_spectrum(scl_dns* d, double* ek, int32_t nbins) {
   if (!d || !ek || nbins < 1 || nbins > d->plan->g.mx) return set_error("bad spectrum arguments"), SCL_EINVAL;
   scl_plan3d* p = d->plan;
@@ -874,14 +1028,172 @@ extern "C" int32_t scl_dns_diagnostics(scl_dns* d, scl_diag* out) {
     using R = decltype(zero);
     std::vector<double> r;
     const int nb = p->g.mx;
-    SCL_CHECK(diag_run<R>(p, (const cpx<R>*)d->fields, nb, r));
+    const cpx<R>* v = (const cpx<R>*)d->fields;
+    SCL_CHECK(diag_run<R>(p, v, nb, r));
     out->energy = r[2 * nb + 0];
     out->enstrophy = r[2 * nb + 1];
     out->dissipation = 2.0 * d->p.nu * r[2 * nb + 1];
     out->helicity = r[2 * nb + 2];
     out->divrms_spec = std::sqrt(r[2 * nb + 3]);
     out->nonfinite = r[2 * nb + 4] > 0 ? 1 : 0;
-    return divmax_run<R>(p, (const cpx<R>*)d->fields, &out->divmax);
+    out->scalar_variance = 0;
+    out->scalar_dissipation = 0;
+    if (d->nc == 4) {
+      double var, kvar;
+      SCL_CHECK(scalar_diag_run<R>(p, v + 3 * p->fstride(), &var, &kvar));
+      out->scalar_variance = var;
+      out->scalar_dissipation = 2.0 * d->p.nu / d->p.schmidt * kvar;
+    }
+    double c[8];
+    SCL_HIP(hipMemcpy(c, d->coef, sizeof(c), hipMemcpyDeviceToHost));
+    out->cs2 = d->p.sgs ? c[5] : 0.0;
+    return divmax_run<R>(p, v, &out->divmax);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// SGS stress and scalar flux on host arrays
+// Runs the LES part of the extended pipeline on v with only tau as output:
+// tau_ij spectral in W[3..8] after prep -> c2r -> phys -> r2c.
+template <class R>
+static int32_t tau_static(scl_plan3d* p, cpx<R>* W, int64_t fs, double cs2, double delta, double* coefdev) {
+  // W[0..2] holds v on entry (as the stage state of the prep kernel)
+  const Geom& g = p->g;
+  const ExtLayout L{1, 0};
+  cpx<R>* st = W + 13 * fs;  // copy of v past the 13 work fields
+  SCL_HIP(hipMemcpyAsync(st, W, 3 * fs * sizeof(cpx<R>), hipMemcpyDeviceToDevice, p->stream));
+  double c0[8] = {0, 0, 0, 0, 0, cs2, 0, 0};
+  SCL_HIP(hipMemcpyAsync(coefdev, c0, sizeof(c0), hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL((k_ext_prep<R>), ExtOps<R>::grid_all(g), dim3(256), 0, p->stream, st, W, fs, g, L);
+  SCL_CHECK(check_launch("ext_prep"));
+  SCL_CHECK(Ops<R>::c2r3d(*p, W, fs, L.nin()));
+  const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
+  hipLaunchKernelGGL((k_ext_phys<R>), ExtOps<R>::grid_main(g), dim3(256), 0, p->stream, W, fs, g, L, scale,
+                     (R)(delta * delta), coefdev + 5);
+  SCL_CHECK(check_launch("ext_phys"));
+  return Ops<R>::r2c3d(*p, W + L.tau_out() * fs, fs, 6);
+}
+
+extern "C" int32_t scl_smag(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* t11, void* t12,
+                            void* t13, void* t22, void* t23, void* t33, const void* kx, const void* ky, const void* kz,
+                            int32_t lx1, int32_t ly, int32_t lz, int32_t nx, int32_t ny, int32_t nz, double delta) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  if (nx != p->g.nx || ny != p->g.ny || nz != p->g.nz) return set_error("nx,ny,nz do not match the plan"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k1d<R>(p, kx, ky, kz));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 16, &s));
+    cpx<R>* W = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    const void* vi[3] = {vx, vy, vz};
+    for (int q = 0; q < 3; ++q) SCL_CHECK(upload<R>(p, vi[q], W + q * fs));
+    // smag.f90:7: c1 = 0.026 (a real(sp) parameter)
+    SCL_CHECK(tau_static<R>(p, W, fs, (double)(R)0.026f, delta, p->coef));
+    void* to[6] = {t11, t12, t13, t22, t23, t33};
+    for (int q = 0; q < 6; ++q) SCL_CHECK(download<R>(p, W + (3 + q) * fs, to[q]));
+    return SCL_OK;
+  });
+}
+
+// kx, ky, kz given as 3D arrays (dynsmag.f90:14): spot-check they are wavenumber()'s
+template <class R>
+static int32_t check_k3d(const scl_plan3d* p, const void* kx, const void* ky, const void* kz) {
+  const R* a[3] = {(const R*)kx, (const R*)ky, (const R*)kz};
+  if (!a[0] || !a[1] || !a[2]) return set_error("null wavenumber array"), SCL_EINVAL;
+  const Geom& g = p->g;
+  const int pts[5][3] = {{0, 0, 0}, {g.mx, g.ny - 1, g.nz - 1}, {1, g.ny / 2, 3 % g.nz}, {2, 1, g.nz / 2}, {g.mx / 3, 5 % g.ny, g.nz - 2}};
+  for (auto& q : pts) {
+    const int64_t o = ((int64_t)q[2] * g.ny + q[1]) * (g.mx + 1) + q[0];
+    const R want[3] = {(R)q[0], (R)kwave(q[1], g.ny), (R)kwave(q[2], g.nz)};
+    for (int c = 0; c < 3; ++c)
+      if (a[c][o] != want[c]) return set_error("3D wavenumber array %d is not wavenumber()'s", c), SCL_EINVAL;
+  }
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_dynsmag(scl_plan3d* p, const void* ux, const void* uy, const void* uz, void* t11, void* t12,
+                               void* t13, void* t22, void* t23, void* t33, const void* kx, const void* ky,
+                               const void* kz, const void* g_test, int32_t lx1, int32_t ly, int32_t lz, int32_t nx,
+                               int32_t ny, int32_t nz, int32_t lx, double delta, double delta_test, double* mm,
+                               double* lm, double* cs2, int32_t update) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  if (nx != p->g.nx || ny != p->g.ny || nz != p->g.nz || lx != p->g.mx)
+    return set_error("nx,ny,nz,lx do not match the plan"), SCL_EINVAL;
+  if (!cs2 || (update && (!mm || !lm))) return set_error("null mm/lm/cs2"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k3d<R>(p, kx, ky, kz));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 40, &s));
+    cpx<R>* W = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    const void* vi[3] = {ux, uy, uz};
+    for (int q = 0; q < 3; ++q) SCL_CHECK(upload<R>(p, vi[q], W + q * fs));
+    if (update) {
+      if (!g_test) return set_error("null g_test"), SCL_EINVAL;
+      // scratch: velocity W[37..39], 24 dynamic fields W[13..36], g_test
+      // staged in W[4] (reference layout) and W[3] (split layout), as reals
+      cpx<R>* V = W + 37 * fs;
+      cpx<R>* D = W + 13 * fs;
+      R* gsplit = (R*)(W + 3 * fs);
+      R* gref = (R*)(W + 4 * fs);
+      SCL_HIP(hipMemcpyAsync(V, W, 3 * fs * sizeof(cpx<R>), hipMemcpyDeviceToDevice, p->stream));
+      SCL_HIP(hipMemcpyAsync(gref, g_test, (size_t)(p->g.mx + 1) * p->g.nrow * sizeof(R), hipMemcpyHostToDevice,
+                             p->stream));
+      hipLaunchKernelGGL((k_ref_to_split_real<R>), dim3((unsigned)((p->g.nrow * (p->g.mx + 1) + 255) / 256)),
+                         dim3(256), 0, p->stream, gref, gsplit, p->g);
+      SCL_CHECK(check_launch("ref_to_split_real"));
+      void* part;
+      SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * (2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &part));
+      SCL_CHECK(ExtOps<R>::dyn_update(*p, V, D, fs, gsplit, (R)delta, (R)delta_test, p->coef, (double*)part));
+      // tau = -cs2 * FFT(2 D^2 |S| S_ij)/N^3 (dynsmag.f90:186-193), from D[0..5]
+      hipLaunchKernelGGL((k_tau_from_s<R>), ExtOps<R>::grid_all(p->g), dim3(256), 0, p->stream, D, fs, p->g,
+                         (const double*)p->coef);
+      SCL_CHECK(check_launch("tau_from_s"));
+      double c[8];
+      SCL_HIP(hipMemcpyAsync(c, p->coef, sizeof(c), hipMemcpyDeviceToHost, p->stream));
+      SCL_HIP(hipStreamSynchronize(p->stream));
+      *cs2 = c[5];
+      *mm = c[6];
+      *lm = c[7];
+      void* to[6] = {t11, t12, t13, t22, t23, t33};
+      for (int q = 0; q < 6; ++q) SCL_CHECK(download<R>(p, D + q * fs, to[q]));
+      return SCL_OK;
+    }
+    SCL_CHECK(tau_static<R>(p, W, fs, *cs2, delta, p->coef));
+    void* to[6] = {t11, t12, t13, t22, t23, t33};
+    for (int q = 0; q < 6; ++q) SCL_CHECK(download<R>(p, W + (3 + q) * fs, to[q]));
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_convecphi_dns(scl_plan3d* p, const void* vx, const void* vy, const void* vz, const void* phi,
+                                     void* wphi, const void* kx, const void* ky, const void* kz, int32_t lx1,
+                                     int32_t ly, int32_t lz, int32_t nx, int32_t ny, int32_t nz) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  if (nx != p->g.nx || ny != p->g.ny || nz != p->g.nz) return set_error("nx,ny,nz do not match the plan"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k1d<R>(p, kx, ky, kz));
+    void* s;
+    SCL_CHECK(plan_scratch(p, 4, &s));
+    cpx<R>* W = (cpx<R>*)s;
+    const int64_t fs = p->fstride();
+    const void* vi[4] = {vx, vy, vz, phi};
+    for (int q = 0; q < 4; ++q) SCL_CHECK(upload<R>(p, vi[q], W + q * fs));
+    SCL_CHECK(Ops<R>::c2r3d(*p, W, fs, 4));
+    const Geom& g = p->g;
+    hipLaunchKernelGGL((k_flux<R>), ExtOps<R>::grid_main(g), dim3(256), 0, p->stream, W, fs,
+                       (R)1 / (R)((double)g.nx * g.ny * g.nz), g);
+    SCL_CHECK(check_launch("flux"));
+    SCL_CHECK(Ops<R>::r2c3d(*p, W, fs, 3));
+    hipLaunchKernelGGL((k_divflux<R>), ExtOps<R>::grid_all(g), dim3(256), 0, p->stream, W, fs, g);
+    SCL_CHECK(check_launch("divflux"));
+    return download<R>(p, W, wphi);
   });
 }
 

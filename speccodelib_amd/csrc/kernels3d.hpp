@@ -23,7 +23,7 @@ namespace scl {
 
 template <class R>
 struct FieldPtrs {
-  cpx<R>* p[6];
+  cpx<R>* p[8];
   int n;
 };
 

@@ -1,0 +1,495 @@
+// Kernels of the LES + passive-scalar RHS and of the dynamic Smagorinsky
+// coefficient.  These run as separate passes around multi-field 3D
+// transforms (prep -> c2r -> physical pointwise -> r2c -> combine), unlike
+// the fused five-pass DNS RHS.
+//
+// Physical-space fields are the main rows in packed form: entry i holds the
+// reals x[2n] (.x) and x[2n+1] (.y); pointwise products act on .x and .y
+// separately, as the reference's packed-real statements do
+// (dynsmag.f90:43-48, smag.f90:44-57).
+#pragma once
+
+#include "kernels3d.hpp"
+
+namespace scl {
+
+// Field counts of the extended RHS: inputs u(3) w(3) [S_ij(6)] [phi(1)],
+// outputs u x w(3) [tau_ij(6)] [u_j phi(3)].
+struct ExtLayout {
+  int les, scalar;
+  __host__ __device__ int nin() const { return 6 + 6 * les + scalar; }
+  __host__ __device__ int nout() const { return 3 + 6 * les + 3 * scalar; }
+  __host__ __device__ int s_in() const { return 6; }                 // S_ij inputs
+  __host__ __device__ int phi_in() const { return 6 + 6 * les; }     // phi input
+  __host__ __device__ int tau_out() const { return 3; }              // tau_ij outputs
+  __host__ __device__ int flux_out() const { return 3 + 6 * les; }   // u_j phi outputs
+};
+
+__device__ inline void ks_of(const Geom& g, int64_t i, int& x, int& y, int& z, int& kx, int& ky, int& kz) {
+  split_xyz(g, i, x, y, z);
+  kx = x;
+  ky = kwave(y, g.ny);
+  kz = kwave(z, g.nz);
+}
+
+// S_ij = (i/2)(k_i v_j + k_j v_i), order 11 12 13 22 23 33 (dynsmag.f90:198-203)
+template <class R>
+__device__ inline void strain_of(const cpx<R> (&v)[3], R kx, R ky, R kz, cpx<R> (&s)[6]) {
+  const R k[3] = {kx, ky, kz};
+  const int P[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int i = P[q][0], j = P[q][1];
+    if (i == j) s[q] = mul_i(k[i] * v[i]);
+    else s[q] = mul_i((R)0.5 * (k[i] * v[j] + k[j] * v[i]));
+  }
+}
+
+// Spectral inputs of the extended RHS from the stage state st (velocity at
+// st, st+fs, st+2fs; scalar at st+3fs).
+template <class R>
+__global__ __launch_bounds__(256) void k_ext_prep(const cpx<R>* __restrict__ st, cpx<R>* __restrict__ W, int64_t fs,
+                                                  Geom g, ExtLayout L) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  const R kx = (R)ikx, ky = (R)iky, kz = (R)ikz;
+  const cpx<R> v[3] = {st[i], st[i + fs], st[i + 2 * fs]};
+  W[i] = v[0];
+  W[i + fs] = v[1];
+  W[i + 2 * fs] = v[2];
+  W[i + 3 * fs] = mul_i(ky * v[2] - kz * v[1]);
+  W[i + 4 * fs] = mul_i(kz * v[0] - kx * v[2]);
+  W[i + 5 * fs] = mul_i(kx * v[1] - ky * v[0]);
+  if (L.les) {
+    cpx<R> s[6];
+    strain_of<R>(v, kx, ky, kz, s);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) W[i + (L.s_in() + q) * fs] = s[q];
+  }
+  if (L.scalar) W[i + L.phi_in() * fs] = st[i + 3 * fs];
+}
+
+// |S| = sqrt(2 S_ij S_ij) of one packed component
+template <class R>
+__device__ inline R abs_strain(const R (&s)[6]) {
+  return sqrt((R)2 * (s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + (R)2 * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4])));
+}
+
+// Physical pointwise work, in place on the main rows: u x w, tau_ij =
+// -2 cs2 D^2 |S| S_ij (dynsmag.f90:206-225), u_j phi (convecphi_dns.f90:23,
+// 33, 42), all times scale = 1/(nx ny nz).  cs2 comes from device memory
+// (the dynamic update writes it there).
+template <class R>
+__global__ __launch_bounds__(256) void k_ext_phys(cpx<R>* __restrict__ W, int64_t fs, Geom g, ExtLayout L, R scale,
+                                                  R d2, const double* __restrict__ cs2p) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nmain) return;
+  const R c = L.les ? (R)(-2.0 * cs2p[0]) * d2 * scale : (R)0;
+  cpx<R> in[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q)
+    if (q < L.nin()) in[q] = W[i + q * fs];
+  cpx<R> out[12];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {  // the two packed reals
+    auto re = [&](int q) -> R { return h ? in[q].y : in[q].x; };
+    R o[12];
+    const R u0 = re(0), u1 = re(1), u2 = re(2), w0 = re(3), w1 = re(4), w2 = re(5);
+    o[0] = (u1 * w2 - u2 * w1) * scale;
+    o[1] = (u2 * w0 - u0 * w2) * scale;
+    o[2] = (u0 * w1 - u1 * w0) * scale;
+    if (L.les) {
+      R s[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) s[q] = re(L.s_in() + q);
+      const R a = abs_strain<R>(s);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[L.tau_out() + q] = c * a * s[q];
+    }
+    if (L.scalar) {
+      const R p = re(L.phi_in());
+      o[L.flux_out() + 0] = u0 * p * scale;
+      o[L.flux_out() + 1] = u1 * p * scale;
+      o[L.flux_out() + 2] = u2 * p * scale;
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      if (h) out[q].y = o[q];
+      else out[q].x = o[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 12; ++q)
+    if (q < L.nout()) W[i + q * fs] = out[q];
+}
+
+// Parameters of the extended combine kernel.
+template <class R>
+struct ExtParams {
+  RhsParams<R> r;
+  R nu_phi;   // scalar diffusivity nu/Sc
+  R grad;     // mean scalar gradient G (source -G u_z)
+  int sign_phi;  // +1: R = -i k.F (physical advection)
+};
+
+// One IF-RK4 stage update of one field component (see k_zfwd_rhs).
+template <int MODE, class R>
+__device__ inline void rk_update(cpx<R> n, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R ef, R em1, R em2,
+                                 bool cut) {
+  const cpx<R> v = V[o];
+  if constexpr (MODE == 1) {
+    const cpx<R> k = dt * n;
+    DU[o] = k;
+    const cpx<R> w = v + (R)0.5 * k;
+    S[o] = w + em1 * w;
+  } else if constexpr (MODE == 2 || MODE == 3) {
+    const cpx<R> k = mk<R>(dt * n.x / ef, dt * n.y / ef);
+    DU[o] = DU[o] + (R)2 * k;
+    if constexpr (MODE == 2) {
+      const cpx<R> w = v + (R)0.5 * k;
+      S[o] = w + em1 * w;
+    } else {
+      const cpx<R> w = v + k;
+      S[o] = w + em2 * w;
+    }
+  } else {
+    const cpx<R> k = mk<R>(dt * n.x / ef / ef, dt * n.y / ef / ef);
+    const cpx<R> du = DU[o] + k;
+    const cpx<R> w = mk<R>(v.x + du.x / (R)6, v.y + du.y / (R)6);
+    V[o] = cut ? mk<R>(0, 0) : w + em2 * w;
+  }
+}
+
+// Combine (spectral): N = u x w + f - i k_j tau_ij (spectral_les.f90:232-234),
+// symmetrize's zeroing, projection; R_phi = -i k . F - G u_z
+// (spectral_dns_scalar.f90:236,245), zeroing; then the IF-RK4 stage (MODE
+// 1..4) for velocity (factor exp(-nu k2 dt/2)) and scalar (exp(-nu/Sc k2
+// dt/2)), or MODE 0: N and R_phi into W[0..3].
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_ext_combine(cpx<R>* W, int64_t fs, cpx<R>* V, cpx<R>* DU, cpx<R>* S,
+                                                     const cpx<R>* st, const double* __restrict__ coef, ExtParams<R> p,
+                                                     Geom g, ExtLayout L) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  const R kx = (R)ikx, ky = (R)iky, kz = (R)ikz;
+  const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
+  const R tiny = std::numeric_limits<R>::min();
+  const R k2 = ik2 == 0 ? tiny : (R)ik2;
+  cpx<R> a = W[i], b = W[i + fs], d = W[i + 2 * fs];
+  if (p.r.forced && k2 < p.r.fk2) {
+    const R fa = (R)coef[0], fb = (R)coef[1];
+    const cpx<R> sx = st[i], sy = st[i + fs], sz = st[i + 2 * fs];
+    a = a + fa * sx;
+    b = b + fa * sy;
+    d = d + fa * sz;
+    if (p.r.forced == 2) {
+      a = a + fb * mul_i(ky * sz - kz * sy);
+      b = b + fb * mul_i(kz * sx - kx * sz);
+      d = d + fb * mul_i(kx * sy - ky * sx);
+    }
+  }
+  if (L.les) {
+    const int t = L.tau_out();
+    const cpx<R> t11 = W[i + t * fs], t12 = W[i + (t + 1) * fs], t13 = W[i + (t + 2) * fs];
+    const cpx<R> t22 = W[i + (t + 3) * fs], t23 = W[i + (t + 4) * fs], t33 = W[i + (t + 5) * fs];
+    a = a - mul_i(kx * t11 + ky * t12 + kz * t13);
+    b = b - mul_i(kx * t12 + ky * t22 + kz * t23);
+    d = d - mul_i(kx * t13 + ky * t23 + kz * t33);
+  }
+  const bool zero = x == g.mx || y == g.ny / 2 || z == g.nz / 2 || ik2 == 0;
+  if (zero) {
+    a = mk<R>(0, 0);
+    b = a;
+    d = a;
+  }
+  {
+    const R den = kx * kx + ky * ky + kz * kz + tiny;
+    const cpx<R> q = mk<R>((kx * a.x + ky * b.x + kz * d.x) / den, (kx * a.y + ky * b.y + kz * d.y) / den);
+    a = a - kx * q;
+    b = b - ky * q;
+    d = d - kz * q;
+  }
+  cpx<R> rp = mk<R>(0, 0);
+  if (L.scalar) {
+    const int f = L.flux_out();
+    const cpx<R> fx = W[i + f * fs], fy = W[i + (f + 1) * fs], fz = W[i + (f + 2) * fs];
+    rp = (R)(-p.sign_phi) * mul_i(kx * fx + ky * fy + kz * fz) - p.grad * st[i + 2 * fs];
+    if (zero) rp = mk<R>(0, 0);
+  }
+  if (p.r.dealias && k2 >= p.r.kc2) {
+    a = mk<R>(0, 0);
+    b = a;
+    d = a;
+    rp = a;
+  }
+  if constexpr (MODE == 0) {
+    W[i] = a;
+    W[i + fs] = b;
+    W[i + 2 * fs] = d;
+    if (L.scalar) W[i + 3 * fs] = rp;
+  } else {
+    const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
+    const R ef = exp_r(xv), em1 = expm1_r(xv), em2 = expm1_r((R)2 * xv);
+    const bool cut = k2 >= p.r.kc2;
+    rk_update<MODE, R>(a, i, V, DU, S, p.r.dt, ef, em1, em2, cut);
+    rk_update<MODE, R>(b, i + fs, V, DU, S, p.r.dt, ef, em1, em2, cut);
+    rk_update<MODE, R>(d, i + 2 * fs, V, DU, S, p.r.dt, ef, em1, em2, cut);
+    if (L.scalar) {
+      const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
+      rk_update<MODE, R>(rp, i + 3 * fs, V, DU, S, p.r.dt, exp_r(xp), expm1_r(xp), expm1_r((R)2 * xp), cut);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Dynamic coefficient (dynsmag.f90:27-194) building blocks.
+// --------------------------------------------------------------------------
+// D[0..2] = v, D[3..5] = g v  (dynsmag.f90:28-33)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_vel(const cpx<R>* __restrict__ st, const R* __restrict__ gt,
+                                                 cpx<R>* __restrict__ D, int64_t fs, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  const R f = gt[i];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const cpx<R> v = st[i + q * fs];
+    D[i + q * fs] = v;
+    D[i + (3 + q) * fs] = f * v;
+  }
+}
+
+// physical: P[q] = u_i u_j for the six pairs (dynsmag.f90:43-48)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_products(const cpx<R>* __restrict__ U, cpx<R>* __restrict__ P, int64_t fs,
+                                                      Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nmain) return;
+  const cpx<R> u[3] = {U[i], U[i + fs], U[i + 2 * fs]};
+  const int PR[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) P[i + q * fs] = mk<R>(u[PR[q][0]].x * u[PR[q][1]].x, u[PR[q][0]].y * u[PR[q][1]].y);
+}
+
+// spectral: A[q] = g * B[q] * scale for q < n (in place when A == B)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_filter(const cpx<R>* B, cpx<R>* A, int64_t fs, int n,
+                                                    const R* __restrict__ gt, R scale, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  const R f = gt[i] * scale;
+  for (int q = 0; q < n; ++q) A[i + q * fs] = f * B[i + q * fs];
+}
+
+// physical: L[q] -= ut_i ut_j (dynsmag.f90:71-76)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_leonard(cpx<R>* __restrict__ Lf, const cpx<R>* __restrict__ Ut,
+                                                     int64_t fs, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nmain) return;
+  const cpx<R> u[3] = {Ut[i], Ut[i + fs], Ut[i + 2 * fs]};
+  const int PR[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const cpx<R> l = Lf[i + q * fs];
+    Lf[i + q * fs] = mk<R>(l.x - u[PR[q][0]].x * u[PR[q][1]].x, l.y - u[PR[q][0]].y * u[PR[q][1]].y);
+  }
+}
+
+// spectral: A[0..5] = S_ij(v), B[0..5] = g S_ij(v) (dynsmag.f90:79-94)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_strain(const cpx<R>* __restrict__ st, const R* __restrict__ gt,
+                                                    cpx<R>* __restrict__ A, cpx<R>* __restrict__ B, int64_t fs,
+                                                    Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  const cpx<R> v[3] = {st[i], st[i + fs], st[i + 2 * fs]};
+  cpx<R> s[6];
+  strain_of<R>(v, (R)ikx, (R)iky, (R)ikz, s);
+  const R f = gt[i];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    A[i + q * fs] = s[q];
+    B[i + q * fs] = f * s[q];
+  }
+}
+
+// physical, in place: S_ij <- c |S| S_ij (dynsmag.f90:104-117, 133-146)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_smag(cpx<R>* __restrict__ A, int64_t fs, R c, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nmain) return;
+  cpx<R> s[6];
+  R sx[6], sy[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    s[q] = A[i + q * fs];
+    sx[q] = s[q].x;
+    sy[q] = s[q].y;
+  }
+  const R ax = abs_strain<R>(sx), ay = abs_strain<R>(sy);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) A[i + q * fs] = mk<R>(c * ax * s[q].x, c * ay * s[q].y);
+}
+
+// physical: A[q] += B[q], q < 6 (dynsmag.f90:164-169)
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_add(cpx<R>* __restrict__ A, const cpx<R>* __restrict__ B, int64_t fs,
+                                                 Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nmain) return;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) A[i + q * fs] = A[i + q * fs] + B[i + q * fs];
+}
+
+// Grid sums mm = sum M:M and lm = sum L:M (off-diagonals twice) over all
+// physical points (dynsmag.f90:171-184), FP64: per-block partials in a
+// fixed order, then k_diag_final.
+template <class R>
+__global__ __launch_bounds__(256) void k_dyn_dots(const cpx<R>* __restrict__ Lf, const cpx<R>* __restrict__ M,
+                                                  int64_t fs, Geom g, double* __restrict__ part) {
+  double mm = 0, lm = 0;
+  const double w[6] = {1, 2, 2, 1, 2, 1};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < g.nmain; i += (int64_t)gridDim.x * 256) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const cpx<R> m = M[i + q * fs], l = Lf[i + q * fs];
+      mm += w[q] * ((double)m.x * m.x + (double)m.y * m.y);
+      lm += w[q] * ((double)l.x * m.x + (double)l.y * m.y);
+    }
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = mm;
+  red[1][threadIdx.x] = lm;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0][0];
+    part[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+// out[j] = sum over blocks b of part[2b + j], j = 0, 1, in block order
+__global__ void k_sum_pairs(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+  const int j = threadIdx.x;
+  if (j >= 2) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; ++b) s += part[2 * b + j];
+  out[j] = s;
+}
+
+// cs2 = lm / mm from the summed partials, into coef[5..7] = {cs2, mm, lm}
+__global__ void k_dyn_cs2(const double* __restrict__ sums, double* __restrict__ coef) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    coef[6] = sums[0];
+    coef[7] = sums[1];
+    coef[5] = sums[1] / sums[0];
+  }
+}
+
+// tau_ij = -cs2 * D[q] (spectral), cs2 = coef[5]
+template <class R>
+__global__ __launch_bounds__(256) void k_tau_from_s(cpx<R>* __restrict__ D, int64_t fs, Geom g,
+                                                    const double* __restrict__ coef) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  const R c = (R)(-coef[5]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) D[i + q * fs] = c * D[i + q * fs];
+}
+
+// g = 1 where k2 <= kt2 (spectral_les.f90:105-109), in the split layout
+template <class R>
+__global__ __launch_bounds__(256) void k_gtest(R* __restrict__ gt, Geom g, R kt2) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
+  const R k2 = ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2;
+  gt[i] = k2 <= kt2 ? (R)1 : (R)0;
+}
+
+// a real array in the reference layout (lx1, ly, lz) -> split layout
+template <class R>
+__global__ __launch_bounds__(256) void k_ref_to_split_real(const R* __restrict__ ref, R* __restrict__ f, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = g.nrow * (g.mx + 1);
+  if (i >= n) return;
+  const int64_t row = i / (g.mx + 1);
+  const int x = (int)(i % (g.mx + 1));
+  if (x < g.mx) f[row * g.mx + x] = ref[i];
+  else f[g.nmain + row] = ref[i];
+}
+
+// convecphi_dns.f90:26-47 in the reference's sign: W[0] = i k . F where
+// F = W[0..2] (spectral fluxes, already scaled)
+template <class R>
+__global__ __launch_bounds__(256) void k_divflux(cpx<R>* __restrict__ W, int64_t fs, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  W[i] = mul_i((R)ikx * W[i] + (R)iky * W[i + fs] + (R)ikz * W[i + 2 * fs]);
+}
+
+// physical: F_j = u_j phi * scale into W[0..2] from u = W[0..2], phi = W[3]
+template <class R>
+__global__ __launch_bounds__(256) void k_flux(cpx<R>* __restrict__ W, int64_t fs, R scale, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.nmain) return;
+  const cpx<R> p = W[i + 3 * fs];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const cpx<R> u = W[i + q * fs];
+    W[i + q * fs] = mk<R>(u.x * p.x * scale, u.y * p.y * scale);
+  }
+}
+
+}  // namespace scl
+
+namespace scl {
+
+// Scalar variance sum' |phi|^2 and sum' k^2 |phi|^2 (half weight on kx = 0),
+// FP64 per-block partials (2 per block) summed by k_sum_pairs.
+template <class R>
+__global__ __launch_bounds__(256) void k_scalar_diag(const cpx<R>* __restrict__ phi, Geom g, double* __restrict__ part) {
+  double v = 0, d = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < g.ntot; i += (int64_t)gridDim.x * 256) {
+    int x, y, z, kx, ky, kz;
+    ks_of(g, i, x, y, z, kx, ky, kz);
+    const cpx<R> a = phi[i];
+    const double w = (x == 0 ? 0.5 : 1.0) * ((double)a.x * a.x + (double)a.y * a.y);
+    v += w;
+    d += ((double)kx * kx + (double)ky * ky + (double)kz * kz) * w;
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = v;
+  red[1][threadIdx.x] = d;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0][0];
+    part[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+}  // namespace scl

@@ -34,7 +34,15 @@ int32_t plan_diag(scl_plan3d* p, size_t bytes, void** out);
 struct scl_dns {
   scl_plan3d* plan = nullptr;
   scl_dns_params p{};
-  void* fields = nullptr;  // V(3) DU(3) S(3) W(6), each plan->fstride() entries
+  // nc state components (3 velocity [+ 1 scalar]); fields V(nc) DU(nc)
+  // S(nc) W(nw), each plan->fstride() entries
+  int nc = 3, nw = 6;
+  bool ext = false;          // LES and/or scalar: the multi-pass RHS
+  void* fields = nullptr;
+  void* dyn = nullptr;       // dynamic coefficient: 24 scratch fields
+  void* gt = nullptr;        // test filter (real, split layout)
+  double* part = nullptr;    // FP64 partial sums of the dynamic update
+  double* coef = nullptr;    // forcing and Smagorinsky coefficients (8 doubles)
   double time = 0.0;
   int64_t steps = 0;
   hipGraph_t graph = nullptr;
