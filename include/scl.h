@@ -213,6 +213,75 @@ enum {
 };
 int32_t scl_dns_step_timed(scl_dns *d, int32_t nsteps, double *kind_ms, int32_t *kind_count, double *total_ms);
 
+/* ---------------------------------------------------------------------
+ * 2D vorticity ensembles.  Replaces the time loop of spectral_dns_2d.f90
+ * (:94 integrating factor, :161-188 RHS) for many independent members of
+ * several grid sizes at once.  A member's field is the reference's
+ * complex(sp|dp), dimension(lx1, ly) vorticity wz, lx1 = n/2 + 1, ly = n.
+ *
+ * RHS: N = -(i kx (u w)^ + i ky (v w)^) + band forcing eps/(conj(w) + tiny)
+ * on fkmin^2 <= k^2 <= fkmax^2, then symmetrize2d and the 2/3 cut, with
+ * u = i ky w/k^2, v = -i kx w/k^2.  The reference negates the divergence only
+ * inside the forced band (:181-185); here the sign is that of -div(u w)
+ * everywhere.  Step: Lawson IF-RK4 with E = exp(-rate dt/2),
+ * rate = c_hyper k^n_hyper + c_hypo k^-n_hypo + damp (the reference's
+ * (k sigma)^nhyper + (nu/k)^nhypo is c_hyper = sigma^nhyper,
+ * c_hypo = nu^nhypo, damp = 0), in place of the reference's IF-AB2
+ * (:192-219).  Sizes: powers of two in [32, 1024] (FP64: [32, 256]).
+ *
+ * per_member = 0: every size group advances as one batch (one set of
+ * launches per group, the groups on parallel branches of one hipGraph per
+ * step).  per_member = 1: every member is its own solver with its own
+ * launches, for comparison.
+ * ------------------------------------------------------------------- */
+typedef struct scl_ens2d scl_ens2d;
+typedef struct scl_ens2d_params {
+  double dt;
+  double c_hyper;   /* coefficient of k^n_hyper */
+  double c_hypo;    /* coefficient of k^-n_hypo (0: none) */
+  double damp;      /* linear damping rate */
+  double fkmin;     /* forced band fkmin <= |k| <= fkmax */
+  double fkmax;
+  double eps;       /* forcing amplitude (spectral_dns_2d.f90:183) */
+  int32_t n_hyper;
+  int32_t n_hypo;
+  int32_t forced;
+  int32_t per_member;
+  int32_t use_graph;
+} scl_ens2d_params;
+
+/* ngroups groups: sizes[g] x sizes[g] grids, members[g] members each. */
+int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t *sizes, const int32_t *members,
+                         const scl_ens2d_params *params, scl_ens2d **ens);
+int32_t scl_ens2d_destroy(scl_ens2d *ens);
+/* Host copy in/out of member j of group g (reference layout). */
+int32_t scl_ens2d_set_member(scl_ens2d *ens, int32_t group, int32_t member, const void *w);
+int32_t scl_ens2d_get_member(scl_ens2d *ens, int32_t group, int32_t member, void *w);
+/* Seeded random-phase vorticity (initialize.f90:465-500 draw order: one
+ * gasdev per mode, y outer, x inner; symmetrize2d; 2/3 cut), shaped to
+ * E(k) ~ k exp(-(k/k0)^2) and scaled to sum' |w|^2/k^2 = energy.  Member q
+ * in group-major order draws from seed0 - q (seed0 < 0). */
+int32_t scl_ens2d_initialize(scl_ens2d *ens, int32_t seed0, double k0, double energy);
+/* Sets member j of group g to w, evaluates N for it, copies N to rhs. */
+int32_t scl_ens2d_rhs(scl_ens2d *ens, int32_t group, int32_t member, const void *w, void *rhs);
+int32_t scl_ens2d_step(scl_ens2d *ens, int32_t nsteps);
+int32_t scl_ens2d_sync(scl_ens2d *ens);
+/* nsteps steps bracketed by HIP events; *ms = elapsed milliseconds. */
+int32_t scl_ens2d_step_timed(scl_ens2d *ens, int32_t nsteps, double *ms);
+int32_t scl_ens2d_time(scl_ens2d *ens, double *time);
+/* FP64 per member, group-major: out[2q] = sum' |w|^2/k^2 (the reference's
+ * ek, spectral_dns_2d.f90:125-128), out[2q+1] = sum' |w|^2 (kx = 0 halved). */
+int32_t scl_ens2d_diagnostics(scl_ens2d *ens, double *out);
+/* Column schedule of group g: points per thread e (8, 16), columns per
+ * workgroup cw (8, 16), x-pass grid row_grid (0 persistent, 1 one
+ * workgroup per row group). */
+int32_t scl_ens2d_set_schedule(scl_ens2d *ens, int32_t group, int32_t e, int32_t cw, int32_t row_grid);
+int32_t scl_ens2d_get_schedule(scl_ens2d *ens, int32_t group, int32_t *e, int32_t *cw, int32_t *row_grid);
+/* Autotuner: times reps RHS evaluations of every valid schedule of every
+ * group on the device, keeps the fastest; best_ms[g] gets its time per RHS
+ * (may be NULL).  The state is not changed. */
+int32_t scl_ens2d_tune(scl_ens2d *ens, int32_t reps, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,7 +72,7 @@ class Vort2D:
         r = -(1j * KX * fx + 1j * KY * fy)
         if p.forced:
             band = (self.k2 >= p.fkmin**2) & (self.k2 <= p.fkmax**2)
-            r = np.where(band, r + p.eps / (np.conj(w) + s3.tiny("dp")), r)
+            r[band] = r[band] + p.eps / (np.conj(w[band]) + s3.tiny("dp"))
         s3.symmetrize2d(r)
         r[self.k2 >= self.kcut2] = 0
         return r.astype(w.dtype)

@@ -72,6 +72,23 @@ class ScDiag(ctypes.Structure):
     ]
 
 
+class ScEns2DParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("c_hyper", ctypes.c_double),
+        ("c_hypo", ctypes.c_double),
+        ("damp", ctypes.c_double),
+        ("fkmin", ctypes.c_double),
+        ("fkmax", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("n_hyper", ctypes.c_int32),
+        ("n_hypo", ctypes.c_int32),
+        ("forced", ctypes.c_int32),
+        ("per_member", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
 def header_functions() -> list[str]:
     """Names of the functions include/scl.h declares."""
     txt = open(HEADER).read()
@@ -125,6 +142,20 @@ def lib():
             "scl_smag": ([P] + [P] * 12 + [I32] * 6 + [D], I32),
             "scl_dynsmag": ([P] + [P] * 13 + [I32] * 7 + [D, D, P, P, P, I32], I32),
             "scl_convecphi_dns": ([P] + [P] * 8 + [I32] * 6, I32),
+            "scl_ens2d_create": ([I32, I32, P, P, ctypes.POINTER(ScEns2DParams), ctypes.POINTER(P)], I32),
+            "scl_ens2d_destroy": ([P], I32),
+            "scl_ens2d_set_member": ([P, I32, I32, P], I32),
+            "scl_ens2d_get_member": ([P, I32, I32, P], I32),
+            "scl_ens2d_initialize": ([P, I32, D, D], I32),
+            "scl_ens2d_rhs": ([P, I32, I32, P, P], I32),
+            "scl_ens2d_step": ([P, I32], I32),
+            "scl_ens2d_sync": ([P], I32),
+            "scl_ens2d_step_timed": ([P, I32, P], I32),
+            "scl_ens2d_time": ([P, P], I32),
+            "scl_ens2d_diagnostics": ([P, P], I32),
+            "scl_ens2d_set_schedule": ([P, I32, I32, I32, I32], I32),
+            "scl_ens2d_get_schedule": ([P, I32, P, P, P], I32),
+            "scl_ens2d_tune": ([P, I32, P], I32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -365,3 +396,93 @@ class DNS:
         tot = ctypes.c_double()
         _check(lib().scl_dns_step_timed(self.h, n, _ptr(ms), _ptr(cnt), ctypes.byref(tot)))
         return tot.value, {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KIND_NAMES)}
+
+
+class Ensemble2D:
+    """Batched 2D vorticity ensembles (spectral_dns_2d.f90 time loop, IF-RK4).
+
+    ``groups`` is a list of ``(n, members)``; a member's field is the
+    reference's ``complex(sp), dimension(lx1, ly)``, i.e. a numpy array of
+    shape ``(n, n//2 + 1)``.  ``per_member=True`` gives every member its own
+    launches (the comparison case)."""
+
+    def __init__(self, groups, prec=SP, dt=5e-4, c_hyper=1e-5, n_hyper=8, c_hypo=0.0, n_hypo=0, damp=0.1,
+                 forced=True, fkmin=4.0, fkmax=5.0, eps=0.01, per_member=False, use_graph=True):
+        self.groups = [(int(n), int(m)) for n, m in groups]
+        self.prec = prec
+        self.params = ScEns2DParams(dt, c_hyper, c_hypo, damp, fkmin, fkmax, eps, int(n_hyper), int(n_hypo),
+                                    int(forced), int(per_member), int(use_graph))
+        sizes = np.array([n for n, _ in self.groups], np.int32)
+        mem = np.array([m for _, m in self.groups], np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().scl_ens2d_create(prec, len(self.groups), _ptr(sizes), _ptr(mem), ctypes.byref(self.params),
+                                      ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().scl_ens2d_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def shape(self, g):
+        n = self.groups[g][0]
+        return (n, n // 2 + 1)
+
+    def set_member(self, g, j, w):
+        _field(w, self.prec, self.shape(g))
+        _check(lib().scl_ens2d_set_member(self.h, g, j, _ptr(w)))
+
+    def get_member(self, g, j):
+        out = np.zeros(self.shape(g), _cdtype(self.prec))
+        _check(lib().scl_ens2d_get_member(self.h, g, j, _ptr(out)))
+        return out
+
+    def initialize(self, seed0=-1, k0=8.0, energy=0.5):
+        _check(lib().scl_ens2d_initialize(self.h, seed0, k0, energy))
+
+    def rhs(self, g, j, w):
+        """Sets member j of group g to w and returns N(w)."""
+        _field(w, self.prec, self.shape(g))
+        out = np.zeros(self.shape(g), _cdtype(self.prec))
+        _check(lib().scl_ens2d_rhs(self.h, g, j, _ptr(w), _ptr(out)))
+        return out
+
+    def step(self, n=1):
+        _check(lib().scl_ens2d_step(self.h, n))
+
+    def sync(self):
+        _check(lib().scl_ens2d_sync(self.h))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        _check(lib().scl_ens2d_step_timed(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        _check(lib().scl_ens2d_time(self.h, ctypes.byref(t)))
+        return t.value
+
+    def diagnostics(self):
+        """(members, 2) FP64: [sum' |w|^2/k^2, sum' |w|^2] per member, group-major."""
+        tot = sum(m for _, m in self.groups)
+        out = np.zeros((tot, 2))
+        _check(lib().scl_ens2d_diagnostics(self.h, _ptr(out)))
+        return out
+
+    def set_schedule(self, g, e, cw, row_grid=0):
+        _check(lib().scl_ens2d_set_schedule(self.h, g, e, cw, row_grid))
+
+    def get_schedule(self, g):
+        v = [ctypes.c_int32() for _ in range(3)]
+        _check(lib().scl_ens2d_get_schedule(self.h, g, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def tune(self, reps=5):
+        """Autotunes every group's schedule; returns the best ms per RHS per group."""
+        best = np.zeros(len(self.groups))
+        _check(lib().scl_ens2d_tune(self.h, reps, _ptr(best)))
+        return best

@@ -45,7 +45,7 @@ static int32_t by_prec(int prec, F&& f) {
 
 // exp(-2 pi i j / L) for j < L, evaluated in FP64 with octant symmetry so
 // that the quarter- and eighth-turn values are exact images of each other.
-static void twiddles(int L, std::vector<double>& re, std::vector<double>& im) {
+void twiddles(int L, std::vector<double>& re, std::vector<double>& im) {
   re.resize(L);
   im.resize(L);
   const double twopi = 6.283185307179586476925286766559;

@@ -1,0 +1,641 @@
+// C ABI of the batched 2D vorticity ensembles: size groups, per-member
+// storage mapping, the IF-RK4 step as one hipGraph, the schedule autotuner.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "ic.hpp"
+#include "vort2d.hpp"
+
+namespace scl {
+int32_t check_launch(const char* what);
+// exp(-2 pi i j/L), j < L, in FP64 (defined with the plan layer)
+void twiddles(int L, std::vector<double>& re, std::vector<double>& im);
+}
+
+namespace {
+
+using namespace scl;
+
+template <int V>
+using Int = std::integral_constant<int, V>;
+
+// One batch of same-size members: six fields of Geom(n, n, nb) in one
+// allocation: three work fields, the state w, the stage accumulator, the
+// stage vorticity.
+struct Group {
+  int n = 0, nb = 0;
+  Geom g{};
+  const void* tw = nullptr;  // exp(-2 pi i j/n), j < n (owned by the ensemble)
+  void* base = nullptr;
+  int e = 8, cw = 8, row_grid = 0;
+  hipStream_t stream = nullptr;  // branch stream (batched mode)
+  hipEvent_t done = nullptr;
+  int64_t fs() const { return g.ntot; }
+  template <class R>
+  cpx<R>* field(int i) const {
+    return reinterpret_cast<cpx<R>*>(base) + i * fs();
+  }
+};
+
+enum { F_W = 0, F_WS = 3, F_ACC = 4, F_S = 5, NFIELDS = 6 };
+
+// FP64 plans keep one schedule (E = 8, 8 columns) and lines up to 256.
+template <class R, int N, int E, int CW>
+constexpr bool schedule_ok() {
+  if constexpr (sizeof(R) == 8) return N <= 256 && E == 8 && CW == 8 && Col2D<R, N, E, CW>::valid;
+  else return Col2D<R, N, E, CW>::valid;
+}
+
+template <class R, class F>
+int32_t with_col2d(int n, int e, int cw, F&& f) {
+  auto go = [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    auto pick = [&](auto EC, auto CC) -> int32_t {
+      constexpr int E = decltype(EC)::value, CW = decltype(CC)::value;
+      if constexpr (schedule_ok<R, N, E, CW>()) {
+        return f(NC, EC, CC);
+      } else {
+        set_error("schedule e=%d cw=%d is not available for n=%d in this precision", E, CW, N);
+        return SCL_EINVAL;
+      }
+    };
+    if (e == 8 && cw == 8) return pick(Int<8>{}, Int<8>{});
+    if (e == 8 && cw == 16) return pick(Int<8>{}, Int<16>{});
+    if (e == 16 && cw == 8) return pick(Int<16>{}, Int<8>{});
+    if (e == 16 && cw == 16) return pick(Int<16>{}, Int<16>{});
+    set_error("schedule e=%d cw=%d: e and cw must be 8 or 16", e, cw);
+    return SCL_EINVAL;
+  };
+  switch (n) {
+    case 16: return go(Int<16>{});
+    case 32: return go(Int<32>{});
+    case 64: return go(Int<64>{});
+    case 128: return go(Int<128>{});
+    case 256: return go(Int<256>{});
+    case 512: return go(Int<512>{});
+    case 1024: return go(Int<1024>{});
+    default: break;
+  }
+  set_error("2D size %d is not a power of two in [16, 1024]", n);
+  return SCL_EINVAL;
+}
+
+template <class R, class F>
+int32_t with_row2d(int m, F&& f) {
+  switch (m) {
+    case 8: return f(Int<8>{});
+    case 16: return f(Int<16>{});
+    case 32: return f(Int<32>{});
+    case 64: return f(Int<64>{});
+    case 128: return f(Int<128>{});
+    case 256:
+      if constexpr (sizeof(R) == 4) return f(Int<256>{});
+      break;
+    case 512:
+      if constexpr (sizeof(R) == 4) return f(Int<512>{});
+      break;
+    default: break;
+  }
+  set_error("2D size %d is not available in this precision", 2 * m);
+  return SCL_EINVAL;
+}
+
+template <class R>
+bool schedule_valid(int n, int e, int cw) {
+  return with_col2d<R>(n, e, cw, [](auto, auto, auto) -> int32_t { return SCL_OK; }) == SCL_OK;
+}
+
+}  // namespace
+
+struct scl_ens2d {
+  int prec = SCL_SP;
+  scl_ens2d_params p{};
+  std::vector<int> sizes, members, first;  // user groups; first internal group of each
+  std::vector<Group> groups;                // internal groups
+  std::vector<std::pair<int, void*>> tw;    // twiddle table per size
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double* diag = nullptr;
+  int ncu = 256;
+  int64_t steps = 0;
+  double time = 0.0;
+  int total_members() const {
+    int t = 0;
+    for (int m : members) t += m;
+    return t;
+  }
+  size_t esize() const { return prec == SCL_SP ? 2 * sizeof(float) : 2 * sizeof(double); }
+};
+
+namespace {
+
+template <class R>
+Rhs2D<R> rhs_params(const scl_ens2d& e, int n) {
+  Rhs2D<R> r;
+  r.dt = (R)e.p.dt;
+  r.fk2min = (R)(e.p.fkmin * e.p.fkmin);
+  r.fk2max = (R)(e.p.fkmax * e.p.fkmax);
+  r.eps = (R)e.p.eps;
+  const double kc = 2.0 * (n / 2) / 3.0;
+  r.kc2 = (R)(kc * kc);
+  r.forced = e.p.forced;
+  r.rate.c_hyper = (R)e.p.c_hyper;
+  r.rate.c_hypo = (R)e.p.c_hypo;
+  r.rate.damp = (R)e.p.damp;
+  r.rate.n_hyper = e.p.n_hyper;
+  r.rate.n_hypo = e.p.n_hypo;
+  return r;
+}
+
+// Workgroups of the x-pass: persistent (as many as can be resident) or one
+// per row group.
+template <class R, int M>
+int64_t row_grid_size(const scl_ens2d& e, const Group& gr) {
+  using RC = RowCfg<R, M>;
+  static int resident = 0;
+  if (!resident) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_row_adv2d<R, M>, RC::BS, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    resident = per_cu;
+  }
+  const int64_t ngroups = gr.g.nrow / RC::RPW;
+  return gr.row_grid ? ngroups : std::min<int64_t>(ngroups, (int64_t)resident * e.ncu);
+}
+
+// One RHS evaluation (mode 0) or IF-RK4 stage (modes 1..4) of a group.
+template <class R>
+int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st) {
+  using C = cpx<R>;
+  C* W = gr.field<R>(F_W);
+  C* WS = gr.field<R>(F_WS);
+  C* ACC = gr.field<R>(F_ACC);
+  C* S = gr.field<R>(F_S);
+  const C* sin = mode <= 1 ? WS : S;
+  const C* tw = reinterpret_cast<const C*>(gr.tw);
+  const int64_t fs = gr.fs();
+  const Rhs2D<R> rp = rhs_params<R>(e, gr.n);
+  SCL_CHECK(with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
+    constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
+    hipLaunchKernelGGL((k_yinv2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
+                       sin, W, fs, gr.g, tw, 1);
+    return check_launch("yinv2d");
+  }));
+  SCL_CHECK(with_row2d<R>(gr.n / 2, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    const R scale = (R)1 / (R)((double)gr.n * gr.n);
+    hipLaunchKernelGGL((k_row_adv2d<R, M>), dim3((unsigned)row_grid_size<R, M>(e, gr)), dim3(RowCfg<R, M>::BS), 0,
+                       st, W, fs, gr.g, scale, tw, 2, 1);
+    return check_launch("row_adv2d");
+  }));
+  return with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
+    constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
+    hipLaunchKernelGGL((k_yfwd2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
+                       W, fs, WS, ACC, S, sin, mode, rp, gr.g, tw, 1);
+    return check_launch("yfwd2d");
+  });
+}
+
+template <class R>
+int32_t group_step(const scl_ens2d& e, const Group& gr, hipStream_t st) {
+  for (int mode = 1; mode <= 4; ++mode) SCL_CHECK(stage<R>(e, gr, mode, st));
+  return SCL_OK;
+}
+
+// One step of every group.  Batched: the groups run on parallel branches
+// (fork/join through events, which graph capture turns into graph edges).
+int32_t launch_step(scl_ens2d& e) {
+  const bool fork = !e.p.per_member && e.groups.size() > 1;
+  if (fork) SCL_HIP(hipEventRecord(e.fork, e.stream));
+  for (auto& gr : e.groups) {
+    hipStream_t st = fork ? gr.stream : e.stream;
+    if (fork) SCL_HIP(hipStreamWaitEvent(st, e.fork, 0));
+    SCL_CHECK(e.prec == SCL_SP ? group_step<float>(e, gr, st) : group_step<double>(e, gr, st));
+    if (fork) {
+      SCL_HIP(hipEventRecord(gr.done, st));
+      SCL_HIP(hipStreamWaitEvent(e.stream, gr.done, 0));
+    }
+  }
+  return SCL_OK;
+}
+
+void drop_graph(scl_ens2d& e) {
+  if (e.exec) hipGraphExecDestroy(e.exec);
+  if (e.graph) hipGraphDestroy(e.graph);
+  e.exec = nullptr;
+  e.graph = nullptr;
+}
+
+int32_t run_steps(scl_ens2d& e, int nsteps) {
+  if (nsteps < 0) return set_error("nsteps < 0"), SCL_EINVAL;
+  if (e.p.use_graph && !e.exec && nsteps > 0) {
+    SCL_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    const int32_t st = launch_step(e);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e.stream, &g);
+    if (st != SCL_OK) {
+      if (g) hipGraphDestroy(g);
+      return st;
+    }
+    SCL_HIP(ce);
+    e.graph = g;
+    SCL_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    if (e.exec) SCL_HIP(hipGraphLaunch(e.exec, e.stream));
+    else SCL_CHECK(launch_step(e));
+    e.steps += 1;
+    e.time += e.p.dt;
+  }
+  return SCL_OK;
+}
+
+// (user group, member) -> (internal group, member index in it)
+int32_t locate(const scl_ens2d& e, int ug, int j, const Group** gr, int* b) {
+  if (ug < 0 || ug >= (int)e.sizes.size()) return set_error("group %d out of range", ug), SCL_EINVAL;
+  if (j < 0 || j >= e.members[ug]) return set_error("member %d out of range for group %d", j, ug), SCL_EINVAL;
+  if (e.p.per_member) {
+    *gr = &e.groups[e.first[ug] + j];
+    *b = 0;
+  } else {
+    *gr = &e.groups[e.first[ug]];
+    *b = j;
+  }
+  return SCL_OK;
+}
+
+// Host reference layout (lx1, ly) <-> split layout of member b, field f.
+int32_t copy_member(const scl_ens2d& e, const Group& gr, int b, int f, void* host, bool to_device) {
+  const size_t es = e.esize();
+  const int n = gr.n, mx = n / 2, lx1 = mx + 1;
+  char* dbase = reinterpret_cast<char*>(gr.base) + (size_t)f * gr.fs() * es;
+  char* dmain = dbase + (size_t)b * n * mx * es;
+  char* dnyq = dbase + ((size_t)gr.g.nmain + (size_t)b * n) * es;
+  char* h = reinterpret_cast<char*>(host);
+  if (to_device) {
+    SCL_HIP(hipMemcpy2DAsync(dmain, mx * es, h, lx1 * es, mx * es, n, hipMemcpyHostToDevice, e.stream));
+    SCL_HIP(hipMemcpy2DAsync(dnyq, es, h + mx * es, lx1 * es, es, n, hipMemcpyHostToDevice, e.stream));
+  } else {
+    SCL_HIP(hipMemcpy2DAsync(h, lx1 * es, dmain, mx * es, mx * es, n, hipMemcpyDeviceToHost, e.stream));
+    SCL_HIP(hipMemcpy2DAsync(h + mx * es, lx1 * es, dnyq, es, es, n, hipMemcpyDeviceToHost, e.stream));
+  }
+  SCL_HIP(hipStreamSynchronize(e.stream));
+  return SCL_OK;
+}
+
+// Seeded vorticity of one member, computed in FP64 (reference layout).
+void initialize2d(std::vector<double>& w, int n, int32_t seed, double k0, double energy) {
+  const int lx1 = n / 2 + 1, ly = n;
+  w.assign(2 * (size_t)lx1 * ly, 0.0);
+  const double pi = 3.1415926536;  // the reference's literal (constant_sp.f90:20)
+  Ran1 ran(seed);
+  for (int jj = 0; jj < ly; ++jj)
+    for (int ii = 0; ii < lx1; ++ii) {
+      const double t1 = ran.uniform<double>();
+      double t2 = ran.uniform<double>();
+      t2 = 2.0 * pi * t2;
+      gasdev<double>(t1, t2, w[2 * ((size_t)jj * lx1 + ii)], w[2 * ((size_t)jj * lx1 + ii) + 1]);
+    }
+  auto at = [&](int x, int y) -> double* { return &w[2 * ((size_t)y * lx1 + x)]; };
+  // symmetrize2d (symmetrize.f90:17-27)
+  for (int x = 0; x < lx1; ++x) at(x, ly / 2)[0] = at(x, ly / 2)[1] = 0;
+  for (int y = 0; y < ly; ++y) at(lx1 - 1, y)[0] = at(lx1 - 1, y)[1] = 0;
+  at(0, 0)[0] = at(0, 0)[1] = 0;
+  for (int y = ly / 2 + 1; y < ly; ++y) {
+    at(0, y)[0] = at(0, ly - y)[0];
+    at(0, y)[1] = -at(0, ly - y)[1];
+  }
+  const double kc = 2.0 * (n / 2) / 3.0, kc2 = kc * kc;
+  double esum = 0.0;
+  for (int y = 0; y < ly; ++y) {
+    const int ky = kwave(y, ly);
+    for (int x = 0; x < lx1; ++x) {
+      const int ik2 = x * x + ky * ky;
+      const double k2 = ik2 == 0 ? std::numeric_limits<double>::min() : (double)ik2;
+      double* c = at(x, y);
+      if (k2 >= kc2) {
+        c[0] = c[1] = 0;
+        continue;
+      }
+      const double k = std::sqrt(k2), s = std::exp(-k2 / (2 * k0 * k0));
+      c[0] = c[0] * k * s;
+      c[1] = c[1] * k * s;
+      const double a = (c[0] * c[0] + c[1] * c[1]) / k2;
+      esum += x == 0 ? 0.5 * a : a;
+    }
+  }
+  const double f = esum > 0 ? std::sqrt(energy / esum) : 0.0;
+  for (double& v : w) v *= f;
+}
+
+template <class R>
+int32_t upload_member_f64(const scl_ens2d& e, const Group& gr, int b, const std::vector<double>& w) {
+  std::vector<R> h(w.size());
+  for (size_t i = 0; i < w.size(); ++i) h[i] = (R)w[i];
+  return copy_member(e, gr, b, F_WS, h.data(), true);
+}
+
+void destroy_groups(scl_ens2d& e) {
+  for (auto& gr : e.groups) {
+    if (gr.base) hipFree(gr.base);
+    if (gr.stream) hipStreamDestroy(gr.stream);
+    if (gr.done) hipEventDestroy(gr.done);
+  }
+  e.groups.clear();
+  for (auto& t : e.tw) hipFree(t.second);
+  e.tw.clear();
+}
+
+template <class R>
+int32_t upload_twiddles(int n, void** out) {
+  std::vector<double> re, im;
+  twiddles(n, re, im);
+  std::vector<cpx<R>> h(n);
+  for (int j = 0; j < n; ++j) h[j] = mk<R>((R)re[j], (R)im[j]);
+  SCL_HIP(hipMalloc(out, n * sizeof(cpx<R>)));
+  SCL_HIP(hipMemcpy(*out, h.data(), n * sizeof(cpx<R>), hipMemcpyHostToDevice));
+  return SCL_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t* sizes, const int32_t* members,
+                                    const scl_ens2d_params* params, scl_ens2d** out) {
+  if (!out || !sizes || !members || !params) return set_error("null argument"), SCL_EINVAL;
+  *out = nullptr;
+  if (prec != SCL_SP && prec != SCL_DP) return set_error("bad precision %d", prec), SCL_EINVAL;
+  if (ngroups < 1) return set_error("ngroups must be >= 1"), SCL_EINVAL;
+  const int hi = prec == SCL_SP ? 1024 : 256;
+  for (int g = 0; g < ngroups; ++g) {
+    const int n = sizes[g];
+    // n >= 32: the x-pass packs 1024/n rows per workgroup and needs n^2 >= 1024
+    if (n < 32 || n > hi || (n & (n - 1)))
+      return set_error("group %d: size %d must be a power of two in [32, %d]", g, n, hi), SCL_EINVAL;
+    if (members[g] < 1) return set_error("group %d: members must be >= 1", g), SCL_EINVAL;
+  }
+  if (!(params->dt > 0)) return set_error("dt must be > 0"), SCL_EINVAL;
+  if (params->n_hyper < 0 || params->n_hypo < 0) return set_error("negative viscosity order"), SCL_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    hipGetLastError();
+    return set_error("no HIP device"), SCL_ENODEV;
+  }
+  auto* e = new scl_ens2d();
+  e->prec = prec;
+  e->p = *params;
+  auto fail = [&](int32_t st) {
+    scl_ens2d_destroy(e);
+    return st;
+  };
+  {
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      e->ncu = ncu;
+  }
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&e->t0) != hipSuccess ||
+      hipEventCreate(&e->t1) != hipSuccess)
+    return set_error("stream/event creation failed"), fail(SCL_EHIP);
+  const size_t es = e->esize();
+  for (int ug = 0; ug < ngroups; ++ug) {
+    const int n = sizes[ug];
+    e->sizes.push_back(n);
+    e->members.push_back(members[ug]);
+    e->first.push_back((int)e->groups.size());
+    const void* tw = nullptr;
+    for (auto& t : e->tw)
+      if (t.first == n) tw = t.second;
+    if (!tw) {
+      void* d = nullptr;
+      const int32_t st = prec == SCL_SP ? upload_twiddles<float>(n, &d) : upload_twiddles<double>(n, &d);
+      if (st != SCL_OK) return fail(st);
+      e->tw.push_back({n, d});
+      tw = d;
+    }
+    const int copies = params->per_member ? members[ug] : 1;
+    const int nb = params->per_member ? 1 : members[ug];
+    for (int c = 0; c < copies; ++c) {
+      Group gr;
+      gr.n = n;
+      gr.nb = nb;
+      gr.g = make_geom(n, n, nb);
+      gr.tw = tw;
+      gr.e = (prec == SCL_SP && n > 512) ? 16 : 8;
+      gr.cw = 8;
+      const size_t bytes = (size_t)NFIELDS * gr.fs() * es;
+      if (hipMalloc(&gr.base, bytes) != hipSuccess) {
+        hipGetLastError();
+        e->groups.push_back(gr);
+        return set_error("device allocation of %zu bytes failed", bytes), fail(SCL_ENOMEM);
+      }
+      if (hipMemsetAsync(gr.base, 0, bytes, e->stream) != hipSuccess) {
+        e->groups.push_back(gr);
+        return set_error("memset failed"), fail(SCL_EHIP);
+      }
+      if (!params->per_member && ngroups > 1) {
+        if (hipStreamCreateWithFlags(&gr.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&gr.done, hipEventDisableTiming) != hipSuccess) {
+          e->groups.push_back(gr);
+          return set_error("stream creation failed"), fail(SCL_EHIP);
+        }
+      }
+      e->groups.push_back(gr);
+    }
+  }
+  if (hipMalloc((void**)&e->diag, 2 * sizeof(double) * e->total_members()) != hipSuccess)
+    return set_error("device allocation failed"), fail(SCL_ENOMEM);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return set_error("stream sync failed"), fail(SCL_EHIP);
+  *out = e;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_destroy(scl_ens2d* e) {
+  if (!e) return SCL_OK;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  drop_graph(*e);
+  destroy_groups(*e);
+  if (e->diag) hipFree(e->diag);
+  if (e->fork) hipEventDestroy(e->fork);
+  if (e->t0) hipEventDestroy(e->t0);
+  if (e->t1) hipEventDestroy(e->t1);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_set_member(scl_ens2d* e, int32_t group, int32_t member, const void* w) {
+  if (!e || !w) return set_error("null argument"), SCL_EINVAL;
+  const Group* gr;
+  int b;
+  SCL_CHECK(locate(*e, group, member, &gr, &b));
+  return copy_member(*e, *gr, b, F_WS, const_cast<void*>(w), true);
+}
+
+extern "C" int32_t scl_ens2d_get_member(scl_ens2d* e, int32_t group, int32_t member, void* w) {
+  if (!e || !w) return set_error("null argument"), SCL_EINVAL;
+  const Group* gr;
+  int b;
+  SCL_CHECK(locate(*e, group, member, &gr, &b));
+  return copy_member(*e, *gr, b, F_WS, w, false);
+}
+
+extern "C" int32_t scl_ens2d_initialize(scl_ens2d* e, int32_t seed0, double k0, double energy) {
+  if (!e) return set_error("null argument"), SCL_EINVAL;
+  if (!(k0 > 0) || !(energy > 0)) return set_error("k0 and energy must be > 0"), SCL_EINVAL;
+  std::vector<double> w;
+  int q = 0;
+  for (int ug = 0; ug < (int)e->sizes.size(); ++ug)
+    for (int j = 0; j < e->members[ug]; ++j, ++q) {
+      initialize2d(w, e->sizes[ug], seed0 - q, k0, energy);
+      const Group* gr;
+      int b;
+      SCL_CHECK(locate(*e, ug, j, &gr, &b));
+      SCL_CHECK(e->prec == SCL_SP ? upload_member_f64<float>(*e, *gr, b, w) : upload_member_f64<double>(*e, *gr, b, w));
+    }
+  e->steps = 0;
+  e->time = 0.0;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_rhs(scl_ens2d* e, int32_t group, int32_t member, const void* w, void* rhs) {
+  if (!e || !w || !rhs) return set_error("null argument"), SCL_EINVAL;
+  const Group* gr;
+  int b;
+  SCL_CHECK(locate(*e, group, member, &gr, &b));
+  SCL_CHECK(copy_member(*e, *gr, b, F_WS, const_cast<void*>(w), true));
+  SCL_CHECK(e->prec == SCL_SP ? stage<float>(*e, *gr, 0, e->stream) : stage<double>(*e, *gr, 0, e->stream));
+  return copy_member(*e, *gr, b, F_W, rhs, false);
+}
+
+extern "C" int32_t scl_ens2d_step(scl_ens2d* e, int32_t nsteps) {
+  if (!e) return set_error("null argument"), SCL_EINVAL;
+  return run_steps(*e, nsteps);
+}
+
+extern "C" int32_t scl_ens2d_sync(scl_ens2d* e) {
+  if (!e) return set_error("null argument"), SCL_EINVAL;
+  SCL_HIP(hipStreamSynchronize(e->stream));
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_step_timed(scl_ens2d* e, int32_t nsteps, double* ms) {
+  if (!e || !ms) return set_error("null argument"), SCL_EINVAL;
+  SCL_HIP(hipStreamSynchronize(e->stream));
+  SCL_HIP(hipEventRecord(e->t0, e->stream));
+  SCL_CHECK(run_steps(*e, nsteps));
+  SCL_HIP(hipEventRecord(e->t1, e->stream));
+  SCL_HIP(hipEventSynchronize(e->t1));
+  float t = 0;
+  SCL_HIP(hipEventElapsedTime(&t, e->t0, e->t1));
+  *ms = t;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_time(scl_ens2d* e, double* time) {
+  if (!e || !time) return set_error("null argument"), SCL_EINVAL;
+  *time = e->time;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_diagnostics(scl_ens2d* e, double* out) {
+  if (!e || !out) return set_error("null argument"), SCL_EINVAL;
+  int q = 0;
+  for (int ug = 0; ug < (int)e->sizes.size(); ++ug) {
+    const int ng = e->p.per_member ? e->members[ug] : 1;
+    for (int c = 0; c < ng; ++c) {
+      const Group& gr = e->groups[e->first[ug] + c];
+      if (e->prec == SCL_SP)
+        hipLaunchKernelGGL((k_ens_diag<float>), dim3(gr.nb), dim3(256), 0, e->stream,
+                           gr.field<float>(F_WS), gr.g, e->diag + 2 * q);
+      else
+        hipLaunchKernelGGL((k_ens_diag<double>), dim3(gr.nb), dim3(256), 0, e->stream,
+                           gr.field<double>(F_WS), gr.g, e->diag + 2 * q);
+      SCL_CHECK(check_launch("ens_diag"));
+      q += gr.nb;
+    }
+  }
+  SCL_HIP(hipMemcpyAsync(out, e->diag, 2 * sizeof(double) * q, hipMemcpyDeviceToHost, e->stream));
+  SCL_HIP(hipStreamSynchronize(e->stream));
+  return SCL_OK;
+}
+
+static int32_t apply_schedule(scl_ens2d* e, int ug, int ee, int cw, int row_grid) {
+  const int n = e->sizes[ug];
+  const bool ok = e->prec == SCL_SP ? schedule_valid<float>(n, ee, cw) : schedule_valid<double>(n, ee, cw);
+  if (!ok) return SCL_EINVAL;
+  const int ng = e->p.per_member ? e->members[ug] : 1;
+  for (int c = 0; c < ng; ++c) {
+    Group& gr = e->groups[e->first[ug] + c];
+    gr.e = ee;
+    gr.cw = cw;
+    gr.row_grid = row_grid ? 1 : 0;
+  }
+  drop_graph(*e);
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_set_schedule(scl_ens2d* e, int32_t group, int32_t ee, int32_t cw, int32_t row_grid) {
+  if (!e) return set_error("null argument"), SCL_EINVAL;
+  if (group < 0 || group >= (int)e->sizes.size()) return set_error("group %d out of range", group), SCL_EINVAL;
+  SCL_HIP(hipStreamSynchronize(e->stream));
+  return apply_schedule(e, group, ee, cw, row_grid);
+}
+
+extern "C" int32_t scl_ens2d_get_schedule(scl_ens2d* e, int32_t group, int32_t* ee, int32_t* cw, int32_t* row_grid) {
+  if (!e || !ee || !cw || !row_grid) return set_error("null argument"), SCL_EINVAL;
+  if (group < 0 || group >= (int)e->sizes.size()) return set_error("group %d out of range", group), SCL_EINVAL;
+  const Group& gr = e->groups[e->first[group]];
+  *ee = gr.e;
+  *cw = gr.cw;
+  *row_grid = gr.row_grid;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_ens2d_tune(scl_ens2d* e, int32_t reps, double* best_ms) {
+  if (!e) return set_error("null argument"), SCL_EINVAL;
+  if (reps < 1) return set_error("reps must be >= 1"), SCL_EINVAL;
+  SCL_HIP(hipStreamSynchronize(e->stream));
+  for (int ug = 0; ug < (int)e->sizes.size(); ++ug) {
+    const int n = e->sizes[ug];
+    const int ng = e->p.per_member ? e->members[ug] : 1;
+    double best = 1e300;
+    int be = -1, bc = -1, br = -1;
+    for (int ee : {8, 16})
+      for (int cw : {8, 16})
+        for (int rg : {0, 1}) {
+          if (apply_schedule(e, ug, ee, cw, rg) != SCL_OK) continue;
+          // time the group's RHS (mode 0 writes only work fields)
+          auto rhs_all = [&]() -> int32_t {
+            for (int c = 0; c < ng; ++c) {
+              const Group& gr = e->groups[e->first[ug] + c];
+              SCL_CHECK(e->prec == SCL_SP ? stage<float>(*e, gr, 0, e->stream) : stage<double>(*e, gr, 0, e->stream));
+            }
+            return SCL_OK;
+          };
+          SCL_CHECK(rhs_all());
+          SCL_HIP(hipEventRecord(e->t0, e->stream));
+          for (int r = 0; r < reps; ++r) SCL_CHECK(rhs_all());
+          SCL_HIP(hipEventRecord(e->t1, e->stream));
+          SCL_HIP(hipEventSynchronize(e->t1));
+          float t = 0;
+          SCL_HIP(hipEventElapsedTime(&t, e->t0, e->t1));
+          const double per = (double)t / reps;
+          if (per < best) {
+            best = per;
+            be = ee;
+            bc = cw;
+            br = rg;
+          }
+        }
+    if (be < 0) return set_error("no valid schedule for size %d", n), SCL_EINVAL;
+    SCL_CHECK(apply_schedule(e, ug, be, bc, br));
+    if (best_ms) best_ms[ug] = best;
+  }
+  return SCL_OK;
+}

@@ -42,6 +42,7 @@ struct Tile {
   int64_t rs;    // row stride
   int kx;        // x index of the column (mx for the kx = nx/2 plane)
   int oth;       // the other batch index: z for the y-pass, y for the z-pass
+  bool ok;       // false for the unused lanes of a partial tail tile
 };
 
 template <int CW, int AXIS>
@@ -56,9 +57,11 @@ __device__ inline Tile col_tile(const Geom& g, int wg, int c) {
       t.rs = g.mx;
       t.kx = xt * CW + c;
       t.oth = z;
+      t.ok = true;
     } else {
       const int z = (wg - nmain) * CW + c;
-      t.base = g.nmain + (int64_t)z * g.ny;
+      t.ok = z < g.nz;
+      t.base = g.nmain + (int64_t)(t.ok ? z : 0) * g.ny;
       t.rs = 1;
       t.kx = g.mx;
       t.oth = z;
@@ -71,9 +74,11 @@ __device__ inline Tile col_tile(const Geom& g, int wg, int c) {
       t.rs = (int64_t)g.ny * g.mx;
       t.kx = xt * CW + c;
       t.oth = y;
+      t.ok = true;
     } else {
       const int y = (wg - nmain) * CW + c;
-      t.base = g.nmain + y;
+      t.ok = y < g.ny;
+      t.base = g.nmain + (t.ok ? y : 0);
       t.rs = g.ny;
       t.kx = g.mx;
       t.oth = y;
@@ -82,9 +87,11 @@ __device__ inline Tile col_tile(const Geom& g, int wg, int c) {
   return t;
 }
 
+// Main tiles plus tail tiles; a partial last tail tile (batch count not a
+// multiple of CW, as for one 2D member) has lanes with ok = false.
 template <int CW, int AXIS>
 inline int col_grid(const Geom& g) {
-  return AXIS == 1 ? (g.mx / CW) * g.nz + g.nz / CW : (g.mx / CW) * g.ny + g.ny / CW;
+  return AXIS == 1 ? (g.mx / CW) * g.nz + (g.nz + CW - 1) / CW : (g.mx / CW) * g.ny + (g.ny + CW - 1) / CW;
 }
 
 // LDS addressing of the Stockham exchanges.  One padding slot after every
@@ -145,8 +152,9 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
   const Tile t = col_tile<CW, AXIS>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
   cpx<R> v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = f[t.base + (int64_t)(rt + T * m) * t.rs];
+  for (int m = 0; m < E; ++m) v[m] = t.ok ? f[t.base + (int64_t)(rt + T * m) * t.rs] : mk<R>(0, 0);
   Line<R, N, E, DIR>::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
+  if (!t.ok) return;
 #pragma unroll
   for (int m = 0; m < E; ++m) f[t.base + (int64_t)Line<R, N, E, DIR>::out_pos(m, rt) * t.rs] = v[m];
 }

@@ -1,0 +1,344 @@
+// 2D vorticity ensembles (spectral_dns_2d.f90:94,161-188).
+//
+// B members of one N x N periodic grid are stored as one split-layout field
+// of Geom(N, N, B): member b takes the place of the z index, so member b's
+// row y is main row b*N + y and its kx = N/2 entry is nyq[b][y].  One RHS
+// evaluation is three passes over the batch:
+//   k_yinv2d     y-inverse of u = i ky w/k^2, v = -i kx w/k^2 and w
+//                (spectral_dns_2d.f90:161-162,166-168, y part)
+//   k_row_adv2d  per row: c2r of u, v, w (x part), the products u w and v w
+//                scaled by 1/N^2, r2c of both (x part) (:166-175)
+//   k_yfwd2d     y-forward of both products, N = -(i kx F + i ky G) plus
+//                the band forcing eps/(conj w + tiny) (:177-185), the
+//                pointwise part of symmetrize2d and the 2/3 cut (:188), the
+//                kx = 0 Hermitian copy, then one Lawson IF-RK4 stage.
+// The reference negates the divergence only inside the forced band (:183),
+// which flips the advection sign elsewhere; here N = -div(u w) + f on every
+// mode.  Its stepper is IF-AB2 (:192-219) on a field that the in-place c2r
+// has already overwritten; the ensembles step with IF-RK4 instead.
+#pragma once
+
+#include <limits>
+
+#include "kernels3d.hpp"
+
+namespace scl {
+
+// Column shape of the y-passes: CW columns of N points, E points per thread.
+template <class R, int N, int E, int CW>
+struct Col2D {
+  static constexpr int T = N / E;
+  static constexpr int BS = CW * T;
+  static constexpr size_t lds_bytes = (size_t)(lds_pad(N) * CW + N) * sizeof(cpx<R>);
+  // radix schedules exist for E = 8 up to N = 512 and E = 16 up to 1024
+  static constexpr bool valid = (E == 8 ? N <= 512 : (E == 16 && N >= 16 && N <= 1024)) && BS <= 1024 &&
+                                BS >= 1 && lds_bytes <= 160 * 1024 && CW <= N / 2;
+};
+
+// Integer power by squaring (exponent >= 0).
+template <class R>
+__device__ inline R ipow(R b, int e) {
+  R r = (R)1;
+  while (e > 0) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return r;
+}
+
+// Damping rate of a mode: c_hyper k^n_hyper + c_hypo k^-n_hypo + damp, the
+// reference's (k sigma)^nhyper + (nu/k)^nhypo (spectral_dns_2d.f90:94) with
+// c_hyper = sigma^nhyper, c_hypo = nu^nhypo, plus linear damping.
+template <class R>
+struct Rate2D {
+  R c_hyper, c_hypo, damp;
+  int n_hyper, n_hypo;
+  __device__ inline R operator()(R k2) const {
+    auto kpow = [&](int n) -> R { return (n & 1) ? ipow(sqrt(k2), n) : ipow(k2, n / 2); };
+    R r = damp;
+    if (c_hyper != (R)0) r += c_hyper * kpow(n_hyper);
+    if (c_hypo != (R)0) r += c_hypo / kpow(n_hypo);
+    return r;
+  }
+};
+
+template <class R>
+struct Rhs2D {
+  R dt;
+  R fk2min, fk2max;  // forced band fkmin^2 <= k^2 <= fkmax^2
+  R eps;             // forcing amplitude
+  R kc2;             // 2/3 cut: zero where k^2 >= kc2
+  int forced;
+  Rate2D<R> rate;
+};
+
+// YINV.  s: the stage vorticity; w: three work fields (u, v, w) at stride fstride.
+template <class R, int N, int E, int CW>
+__global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yinv2d(const cpx<R>* __restrict__ s,
+                                                                   cpx<R>* __restrict__ w, int64_t fstride, Geom g,
+                                                                   const cpx<R>* __restrict__ tw, int tws) {
+  using CC = Col2D<R, N, E, CW>;
+  constexpr int T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 1>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  const R kx = (R)t.kx;
+  const R tiny = std::numeric_limits<R>::min();
+  using L = Line<R, N, E, +1>;
+  cpx<R> u[E], v[E], o[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int y = rt + T * m;
+    o[m] = t.ok ? s[t.base + (int64_t)y * t.rs] : mk<R>(0, 0);
+    const int iky = kwave(y, N);
+    const int ik2 = t.kx * t.kx + iky * iky;
+    const R k2 = ik2 == 0 ? tiny : (R)ik2;
+    const R ky = (R)iky;
+    // u = i ky w/k^2, v = -i kx w/k^2 (spectral_dns_2d.f90:161-162)
+    u[m] = mk<R>(-(ky * o[m].y) / k2, (ky * o[m].x) / k2);
+    v[m] = mk<R>((kx * o[m].y) / k2, -(kx * o[m].x) / k2);
+  }
+  L::run(u, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(o, lds, rt, twl, 1, ColIdx<CW>{c});
+  if (!t.ok) return;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int64_t off = t.base + (int64_t)L::out_pos(m, rt) * t.rs;
+    w[off] = u[m];
+    w[off + fstride] = v[m];
+    w[off + 2 * fstride] = o[m];
+  }
+}
+
+// XPASS: per row, c2r of u, v, w; u w and v w times scale; r2c of both
+// into work fields 0 and 1.  Persistent grid over groups of RPW rows.
+template <class R, int M>
+__global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_adv2d(cpx<R>* W, int64_t fstride, Geom g, R scale,
+                                                                const cpx<R>* __restrict__ tw, int twsM, int twsN) {
+  using C = RowCfg<R, M>;
+  constexpr int T = C::TPR;
+  __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
+  __shared__ cpx<R> twM[M], twN[M];
+  for (int i = threadIdx.x; i < M; i += C::BS) {
+    twM[i] = tw[i * twsM];
+    twN[i] = tw[i * twsN];
+  }
+  __syncthreads();
+  const int slot = threadIdx.x / T, rt = threadIdx.x % T;
+  cpx<R>* lrow = lds + slot * lds_pad(M);
+  using L = Line<R, M, 8, +1>;
+  using LF = Line<R, M, 8, -1>;
+  const int64_t ngroups = g.nrow / C::RPW;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row = grp * C::RPW + slot;
+    cpx<R> raw[3][8], xm[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const cpx<R>* base = W + f * fstride;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) raw[f][m] = base[row * M + rt + T * m];
+      xm[f] = rt == 0 ? base[g.nmain + row] : mk<R>(0, 0);
+    }
+    cpx<R> a[3][8];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      row_c2r_prep<R, M>(a[f], raw[f], xm[f], rt, twN, 1);
+      L::run(a[f], lrow, rt, twM, 1, LinIdx{});
+    }
+    // real and imaginary parts are the even and odd x points
+    // (spectral_dns_2d.f90:170-171)
+    cpx<R> p[2][8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const cpx<R> uu = a[0][m], vv = a[1][m], ww = a[2][m];
+      p[0][m] = mk<R>(uu.x * ww.x * scale, uu.y * ww.y * scale);
+      p[1][m] = mk<R>(vv.x * ww.x * scale, vv.y * ww.y * scale);
+    }
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      cpx<R>* base = W + f * fstride;
+      LF::run(p[f], lrow, rt, twM, 1, LinIdx{});
+      row_r2c_post<R, M>(p[f], base + row * M, base + g.nmain + row, rt, twN, 1);
+    }
+  }
+}
+
+// YFWD + stage.  W: the two product fields (in), WS: state w, ACC: stage
+// accumulator, S: stage vorticity (written for the next stage), st: the
+// vorticity of this stage (forcing), equal to WS in mode 1 and to S after.
+// With E = exp(-rate dt/2) and a = dt N (division-free Lawson form):
+//   mode 0: W[0] = N
+//   mode 1: ACC = E^2 (w + a/6);  S = E (w + a/2)
+//   mode 2: ACC += E a/3;         S = E w + a/2
+//   mode 3: ACC += E a/3;         S = E (E w + a)
+//   mode 4: w = ACC + a/6, zero where k^2 >= kc2
+// which is dns-deform.f90:164-237's scheme multiplied through by E, E and
+// E^2 (the reference form divides by E, which underflows under k^8
+// hyperviscosity).  E x is applied as x + expm1(.) x.
+template <class R, int N, int E, int CW>
+__global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, int64_t fstride, cpx<R>* WS,
+                                                                   cpx<R>* ACC, cpx<R>* S, const cpx<R>* st, int mode,
+                                                                   Rhs2D<R> p, Geom g,
+                                                                   const cpx<R>* __restrict__ tw, int tws) {
+  using CC = Col2D<R, N, E, CW>;
+  constexpr int T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const int wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int xtn = g.mx / CW;
+  const bool kx0_tile = wg < xtn * g.nz && wg % xtn == 0;  // uniform over the workgroup
+  const Tile t = col_tile<CW, 1>(g, wg, c);
+  using L = Line<R, N, E, -1>;
+  cpx<R> f0[E], f1[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int64_t o = t.base + (int64_t)(rt + T * m) * t.rs;
+    f0[m] = t.ok ? W[o] : mk<R>(0, 0);
+    f1[m] = t.ok ? W[o + fstride] : mk<R>(0, 0);
+  }
+  L::run(f0, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(f1, lds, rt, twl, 1, ColIdx<CW>{c});
+
+  const R tiny = std::numeric_limits<R>::min();
+  const int ikx = t.kx;
+  const R kx = (R)ikx;
+  cpx<R> o1[E], o2[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int y = L::out_pos(m, rt);
+    const int iky = kwave(y, N);
+    const R ky = (R)iky;
+    const int ik2 = ikx * ikx + iky * iky;
+    const R k2 = ik2 == 0 ? tiny : (R)ik2;
+    const int64_t o = t.base + (int64_t)y * t.rs;
+    // -(i kx F + i ky G) (spectral_dns_2d.f90:177-178)
+    const cpx<R> d = kx * f0[m] + ky * f1[m];
+    cpx<R> r = mk<R>(d.y, -d.x);
+    if (p.forced && t.ok && k2 >= p.fk2min && k2 <= p.fk2max) {
+      // eps/(conj(w) + tiny) (:182-183)
+      const cpx<R> ws = st[o];
+      const R a = ws.x + tiny, b = ws.y;
+      const R den = a * a + b * b;
+      r = r + mk<R>(p.eps * a / den, p.eps * b / den);
+    }
+    // symmetrize2d (symmetrize.f90:17-22) and the 2/3 cut
+    if (y == N / 2 || t.kx == g.mx || ik2 == 0 || k2 >= p.kc2) r = mk<R>(0, 0);
+    if (mode == 0 || !t.ok) {
+      o1[m] = r;
+      o2[m] = mk<R>(0, 0);
+      continue;
+    }
+    const R x = -p.rate(k2) * p.dt / (R)2;
+    const R em1 = expm1_r(x);
+    const cpx<R> wv = WS[o];
+    const cpx<R> a = p.dt * r;
+    if (mode == 1) {
+      const cpx<R> u = wv + ((R)1 / (R)6) * a, q = wv + (R)0.5 * a;
+      o1[m] = u + expm1_r((R)2 * x) * u;
+      o2[m] = q + em1 * q;
+    } else if (mode == 2 || mode == 3) {
+      const cpx<R> q = ((R)1 / (R)3) * a;
+      o1[m] = ACC[o] + (q + em1 * q);
+      const cpx<R> ew = wv + em1 * wv;
+      if (mode == 2) {
+        o2[m] = ew + (R)0.5 * a;
+      } else {
+        const cpx<R> u = ew + a;
+        o2[m] = u + em1 * u;
+      }
+    } else {
+      o1[m] = k2 >= p.kc2 ? mk<R>(0, 0) : ACC[o] + ((R)1 / (R)6) * a;
+    }
+  }
+  // kx = 0 column: c(0, -ky) = conj c(0, ky) (symmetrize.f90:25-27), applied
+  // to the outputs, which equals symmetrizing N first: every output is a
+  // real-linear function of N and of inputs that already have this symmetry.
+  if (kx0_tile) {
+    __syncthreads();
+    if (c == 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int y = L::out_pos(m, rt);
+        lds[y] = o1[m];
+        lds[N + y] = o2[m];
+      }
+    }
+    __syncthreads();
+    if (c == 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int y = L::out_pos(m, rt);
+        if (y > N / 2) {
+          o1[m] = conj(lds[N - y]);
+          o2[m] = conj(lds[2 * N - y]);
+        }
+      }
+    }
+  }
+  if (!t.ok) return;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int64_t o = t.base + (int64_t)L::out_pos(m, rt) * t.rs;
+    if (mode == 0) {
+      W[o] = o1[m];
+    } else if (mode == 4) {
+      WS[o] = o1[m];
+    } else {
+      ACC[o] = o1[m];
+      S[o] = o2[m];
+    }
+  }
+}
+
+// Per-member FP64 sums in a fixed order: out[2b] = sum' |w|^2/k^2 (twice the
+// kinetic energy's half-plane sum, as spectral_dns_2d.f90:125-128 with
+// kx = 0 halved), out[2b+1] = sum' |w|^2.  One workgroup per member.
+template <class R>
+__global__ __launch_bounds__(256) void k_ens_diag(const cpx<R>* __restrict__ w, Geom g, double* __restrict__ out) {
+  const int b = blockIdx.x;
+  const int64_t nm = (int64_t)g.ny * g.mx;
+  double e = 0.0, z = 0.0;
+  for (int64_t i = threadIdx.x; i < nm + g.ny; i += 256) {
+    int kx, y;
+    cpx<R> v;
+    if (i < nm) {
+      kx = (int)(i % g.mx);
+      y = (int)(i / g.mx);
+      v = w[(int64_t)b * nm + i];
+    } else {
+      kx = g.mx;
+      y = (int)(i - nm);
+      v = w[g.nmain + (int64_t)b * g.ny + y];
+    }
+    const int ky = kwave(y, g.ny);
+    const int ik2 = kx * kx + ky * ky;
+    const double k2 = ik2 == 0 ? std::numeric_limits<double>::min() : (double)ik2;
+    const double a = (double)v.x * v.x + (double)v.y * v.y;
+    const double wt = kx == 0 ? 0.5 : 1.0;
+    e += wt * a / k2;
+    z += wt * a;
+  }
+  __shared__ double se[256], sz[256];
+  se[threadIdx.x] = e;
+  sz[threadIdx.x] = z;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      sz[threadIdx.x] += sz[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * b] = se[0];
+    out[2 * b + 1] = sz[0];
+  }
+}
+
+}  // namespace scl
