@@ -282,6 +282,35 @@ int32_t scl_ens2d_get_schedule(scl_ens2d *ens, int32_t group, int32_t *e, int32_
  * (may be NULL).  The state is not changed. */
 int32_t scl_ens2d_tune(scl_ens2d *ens, int32_t reps, double *best_ms);
 
+/* ---------------------------------------------------------------------
+ * Discrete adjoint of nsteps IF-RK4 steps (dns-deform.f90:160-237) of the
+ * unforced momentum equation, DNS (sgs = 0) or static Smagorinsky LES
+ * (sgs = 1: spectral_les.f90:205-238 with dynsmag.f90:196-233's static
+ * branch), on the plan's grid and precision.  No counterpart in the
+ * reference, which has no adjoint; the forward map is scl_dns_step's.
+ * Gradients are in the real inner product of the stored half spectrum:
+ * per stored entry dL/dRe + i dL/dIm.  Checkpoints every ckpt_every steps;
+ * the backward sweep recomputes one segment at a time into a tape of stage
+ * states.  on_device = 1: every array argument is a device pointer (field
+ * in the reference layout), 0: host pointers.
+ * ------------------------------------------------------------------- */
+typedef struct scl_adjoint scl_adjoint;
+int32_t scl_adjoint_create(scl_plan3d *plan, const scl_dns_params *params, int32_t nsteps, int32_t ckpt_every,
+                           scl_adjoint **adj);
+int32_t scl_adjoint_destroy(scl_adjoint *adj);
+/* u(0) -> u(T) with cs2 = C_s^2; keeps the checkpoints for the backward sweep. */
+int32_t scl_adjoint_forward(scl_adjoint *adj, double cs2, const void *u0x, const void *u0y, const void *u0z,
+                            void *uTx, void *uTy, void *uTz, int32_t on_device);
+/* dL/du(T) -> dL/du(0) and dL/dcs2 for the last forward's u(0) and cs2. */
+int32_t scl_adjoint_backward(scl_adjoint *adj, const void *gx, const void *gy, const void *gz, void *g0x, void *g0y,
+                             void *g0z, double *dcs2, int32_t on_device);
+/* One vector-Jacobian product of the RHS: out = J(s)^T nbar, *dcs2 = <nbar, dN/dcs2>. */
+int32_t scl_adjoint_vjp(scl_adjoint *adj, double cs2, const void *sx, const void *sy, const void *sz,
+                        const void *nx, const void *ny, const void *nz, void *ox, void *oy, void *oz, double *dcs2,
+                        int32_t on_device);
+/* Event-timed milliseconds of the last forward and backward calls. */
+int32_t scl_adjoint_timing(scl_adjoint *adj, double *forward_ms, double *backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

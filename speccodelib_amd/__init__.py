@@ -156,6 +156,12 @@ def lib():
             "scl_ens2d_set_schedule": ([P, I32, I32, I32, I32], I32),
             "scl_ens2d_get_schedule": ([P, I32, P, P, P], I32),
             "scl_ens2d_tune": ([P, I32, P], I32),
+            "scl_adjoint_create": ([P, ctypes.POINTER(ScDnsParams), I32, I32, ctypes.POINTER(P)], I32),
+            "scl_adjoint_destroy": ([P], I32),
+            "scl_adjoint_forward": ([P, D] + [P] * 6 + [I32], I32),
+            "scl_adjoint_backward": ([P] + [P] * 6 + [P, I32], I32),
+            "scl_adjoint_vjp": ([P, D] + [P] * 9 + [P, I32], I32),
+            "scl_adjoint_timing": ([P, P, P], I32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -486,3 +492,63 @@ class Ensemble2D:
         best = np.zeros(len(self.groups))
         _check(lib().scl_ens2d_tune(self.h, reps, _ptr(best)))
         return best
+
+
+class Adjoint:
+    """Discrete adjoint of nsteps IF-RK4 steps of the unforced momentum
+    equation (DNS, or static Smagorinsky LES with sgs=1) on a plan.
+
+    Fields are numpy arrays in the reference layout (host path) or device
+    pointers (``on_device=True`` with integer addresses, e.g. torch
+    ``tensor.data_ptr()``).  Gradients: dL/dRe + i dL/dIm per stored entry."""
+
+    def __init__(self, plan: Plan3D, nsteps, ckpt_every=20, nu=0.005, dt=0.002, sgs=1, delta=0.0):
+        self.plan = plan
+        self.nsteps = nsteps
+        self.params = ScDnsParams(nu, dt, 0, 0.0, 0.0, 0.0, 0, 0, int(sgs), 0.0, delta, 0.0, 0.0, 10, 0, 0.0, 1.0)
+        h = ctypes.c_void_p()
+        _check(lib().scl_adjoint_create(plan.h, ctypes.byref(self.params), nsteps, ckpt_every, ctypes.byref(h)))
+        self.h = h
+        self._fwd_token = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().scl_adjoint_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _in(self, arrs):
+        for a in arrs:
+            _field(a, self.plan.prec, self.plan.shape)
+        return [_ptr(a) for a in arrs]
+
+    def forward(self, u0, cs2):
+        out = [self.plan.zeros() for _ in range(3)]
+        _check(lib().scl_adjoint_forward(self.h, cs2, *self._in(u0), *map(_ptr, out), 0))
+        return tuple(out)
+
+    def backward(self, gT):
+        out = [self.plan.zeros() for _ in range(3)]
+        d = ctypes.c_double()
+        _check(lib().scl_adjoint_backward(self.h, *self._in(gT), *map(_ptr, out), ctypes.byref(d), 0))
+        return tuple(out), d.value
+
+    def vjp(self, s, nbar, cs2):
+        out = [self.plan.zeros() for _ in range(3)]
+        d = ctypes.c_double()
+        _check(lib().scl_adjoint_vjp(self.h, cs2, *self._in(s), *self._in(nbar), *map(_ptr, out), ctypes.byref(d), 0))
+        return tuple(out), d.value
+
+    def forward_device(self, cs2, u0_ptrs, uT_ptrs):
+        _check(lib().scl_adjoint_forward(self.h, cs2, *u0_ptrs, *uT_ptrs, 1))
+
+    def backward_device(self, gT_ptrs, g0_ptrs):
+        d = ctypes.c_double()
+        _check(lib().scl_adjoint_backward(self.h, *gT_ptrs, *g0_ptrs, ctypes.byref(d), 1))
+        return d.value
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().scl_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value

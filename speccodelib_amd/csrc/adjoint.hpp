@@ -1,0 +1,311 @@
+// Discrete adjoint of the IF-RK4 step (dns-deform.f90:160-237 stepping) of
+// the momentum RHS N(v) = P Sym [ FFT(u x w)/N^3 - i k_j tau_ij ] with the
+// static Smagorinsky stress tau_ij = FFT(-2 cs2 D^2 |S| S_ij)/N^3
+// (spectral_les.f90:205-238, dynsmag.f90:196-233), or without it.
+//
+// Gradients are taken in the real inner product of the stored half
+// spectrum, <a, b> = sum Re(conj(a) b) over every stored entry, so the
+// gradient of an entry is dL/dRe + i dL/dIm.  With that product:
+//   adjoint of c2r (FFTW halfcomplex: Im of the kx = 0 and kx = nx/2 lines
+//     ignored)  = r2c followed by a weight 2 on 0 < kx < nx/2, 1 on kx = 0
+//     and kx = nx/2;
+//   adjoint of r2c = c2r after the inverse weights (1/2 interior, 1 on the
+//     two planes);
+//   multiplication by a complex number a -> by conj(a): the curl i k x and
+//     the projection are self-adjoint;
+//   symmetrize (zero planes, then c(0,-k) = conj c(0,k)) -> for every copy
+//     target t with source s: s += conj(t), t = 0; then zero the planes.
+// One vector-Jacobian product J(s)^T y runs as prep (spectral) -> c2r ->
+// pointwise (physical) -> r2c -> combine (spectral), like the forward RHS.
+#pragma once
+
+#include "les.hpp"
+
+namespace scl {
+
+// Field counts of one vector-Jacobian product.  Inputs to the inverse
+// transforms: u(3) w(3) [S_ij(6)] ybar(3) [taubar_ij(6)]; outputs of the
+// pointwise pass: ubar(3) wbar(3) [Sbar_ij(6)].
+struct AdjLayout {
+  int les;
+  __host__ __device__ int nin() const { return 9 + 12 * les; }
+  __host__ __device__ int nout() const { return 6 + 6 * les; }
+  __host__ __device__ int y_in() const { return 6 + 6 * les; }
+  __host__ __device__ int tau_in() const { return 9 + 6 * les; }
+};
+
+__device__ inline bool sym_zero_mode(const Geom& g, int x, int y, int z, int ik2) {
+  return x == g.mx || y == g.ny / 2 || z == g.nz / 2 || ik2 == 0;
+}
+
+// v <- P v = v - k (k.v)/(k^2 + tiny)  (projection.f90:16-30)
+template <class R>
+__device__ inline void project(cpx<R> (&v)[3], R kx, R ky, R kz) {
+  const R den = kx * kx + ky * ky + kz * kz + std::numeric_limits<R>::min();
+  const cpx<R> q = mk<R>((kx * v[0].x + ky * v[1].x + kz * v[2].x) / den, (kx * v[0].y + ky * v[1].y + kz * v[2].y) / den);
+  v[0] = v[0] - kx * q;
+  v[1] = v[1] - ky * q;
+  v[2] = v[2] - kz * q;
+}
+
+// Spectral inputs of J(s)^T: the stage state's u, w = i k x u, S_ij, and
+// ybar = Z(Y) (Y = P nbar after the Hermitian transpose; Z zeroes the
+// symmetrize planes), then taubar_ij from ybar (adjoint of -i k_j tau_ij),
+// both times the inverse weights of the r2c adjoint.
+template <class R>
+__global__ __launch_bounds__(256) void k_adj_prep(const cpx<R>* __restrict__ s, const cpx<R>* __restrict__ Y,
+                                                  cpx<R>* __restrict__ W, int64_t fs, Geom g, AdjLayout L) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  const R kx = (R)ikx, ky = (R)iky, kz = (R)ikz;
+  const cpx<R> v[3] = {s[i], s[i + fs], s[i + 2 * fs]};
+  W[i] = v[0];
+  W[i + fs] = v[1];
+  W[i + 2 * fs] = v[2];
+  W[i + 3 * fs] = mul_i(ky * v[2] - kz * v[1]);
+  W[i + 4 * fs] = mul_i(kz * v[0] - kx * v[2]);
+  W[i + 5 * fs] = mul_i(kx * v[1] - ky * v[0]);
+  if (L.les) {
+    cpx<R> st[6];
+    strain_of<R>(v, kx, ky, kz, st);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) W[i + (6 + q) * fs] = st[q];
+  }
+  const bool zero = sym_zero_mode(g, x, y, z, ikx * ikx + iky * iky + ikz * ikz);
+  const R wi = (x == 0 || x == g.mx) ? (R)1 : (R)0.5;
+  cpx<R> yb[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) yb[q] = zero ? mk<R>(0, 0) : wi * Y[i + q * fs];
+  const int yo = L.y_in();
+  W[i + yo * fs] = yb[0];
+  W[i + (yo + 1) * fs] = yb[1];
+  W[i + (yo + 2) * fs] = yb[2];
+  if (L.les) {
+    // forward: N_x -= i (kx t11 + ky t12 + kz t13), N_y -= i (kx t12 + ky t22 + kz t23),
+    //          N_z -= i (kx t13 + ky t23 + kz t33)
+    const int t = L.tau_in();
+    W[i + t * fs] = mul_i(kx * yb[0]);
+    W[i + (t + 1) * fs] = mul_i(ky * yb[0] + kx * yb[1]);
+    W[i + (t + 2) * fs] = mul_i(kz * yb[0] + kx * yb[2]);
+    W[i + (t + 3) * fs] = mul_i(ky * yb[1]);
+    W[i + (t + 4) * fs] = mul_i(kz * yb[1] + ky * yb[2]);
+    W[i + (t + 5) * fs] = mul_i(kz * yb[2]);
+  }
+}
+
+// Physical pointwise part, in place on the packed main rows.  With the
+// forward's q = scale u x w and sigma_ij = c |S| S_ij (c = -2 cs2 D^2 scale):
+//   qbar = scale * c2r(ybar);  ubar = w x qbar;  wbar = qbar x u;
+//   Sbar_q = c (|S| sigmabar_q + 2 m_q S_q (sum_p sigmabar_p S_p)/|S|),
+//   m = 1 on the diagonal, 2 off it;
+// and the cs2 derivative sum over points of sigmabar_p dsigma_p/dcs2 =
+// cc |S| sum_p sigmabar_p S_p (cc = -2 D^2 scale), per block into part[].
+template <class R>
+__global__ __launch_bounds__(256) void k_adj_phys(cpx<R>* __restrict__ W, int64_t fs, Geom g, AdjLayout L, R scale,
+                                                  R c, R cc, double* __restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double acc = 0.0;
+  if (i < g.nmain) {
+    cpx<R> in[21];
+#pragma unroll
+    for (int q = 0; q < 21; ++q)
+      if (q < L.nin()) in[q] = W[i + q * fs];
+    cpx<R> out[12];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      auto re = [&](int q) -> R { return h ? in[q].y : in[q].x; };
+      R o[12];
+      const R u0 = re(0), u1 = re(1), u2 = re(2), w0 = re(3), w1 = re(4), w2 = re(5);
+      const int yo = L.y_in();
+      const R q0 = scale * re(yo), q1 = scale * re(yo + 1), q2 = scale * re(yo + 2);
+      o[0] = w1 * q2 - w2 * q1;
+      o[1] = w2 * q0 - w0 * q2;
+      o[2] = w0 * q1 - w1 * q0;
+      o[3] = q1 * u2 - q2 * u1;
+      o[4] = q2 * u0 - q0 * u2;
+      o[5] = q0 * u1 - q1 * u0;
+      if (L.les) {
+        R sv[6], sb[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          sv[q] = re(6 + q);
+          sb[q] = re(L.tau_in() + q);
+        }
+        const R a = abs_strain<R>(sv);
+        R t = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) t += sb[q] * sv[q];
+        const R f = a > (R)0 ? (R)2 * t / a : (R)0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          const R m = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
+          o[6 + q] = c * (a * sb[q] + m * f * sv[q]);
+        }
+        acc += (double)cc * (double)a * (double)t;
+      }
+#pragma unroll
+      for (int q = 0; q < 12; ++q) {
+        if (h) out[q].y = o[q];
+        else out[q].x = o[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      if (q < L.nout()) W[i + q * fs] = out[q];
+  }
+  if (part) {
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  }
+}
+
+// Parameters of the adjoint stage bookkeeping.
+template <class R>
+struct AdjParams {
+  R dt, nu, kc2;
+};
+
+// Reverse of the stage updates (E = exp(-nu k^2 dt/2), M = [k^2 < kc2]):
+// forward  k1 = dt N(v),        s2 = E (v + k1/2)
+//          k2 = dt N(s2)/E,     s3 = E (v + k2/2)
+//          k3 = dt N(s3)/E,     s4 = E^2 (v + k3)
+//          k4 = dt N(s4)/E^2,   v' = M E^2 (v + (k1 + 2 k2 + 2 k3 + k4)/6)
+// reverse  g = M E^2 lam; vbar = g; nbar4 = dt g/(6 E^2)                (init)
+//          sbar4 = J(s4)^T nbar4: vbar += E^2 sbar4; nbar3 = dt (g/3 + E^2 sbar4)/E     (MODE 4)
+//          sbar3:                 vbar += E sbar3;   nbar2 = dt (g/3 + E sbar3/2)/E     (MODE 3)
+//          sbar2:                 vbar += E sbar2;   nbar1 = dt (g/6 + E sbar2/2)       (MODE 2)
+//          sbar1 = J(v)^T nbar1:  vbar += sbar1                                          (MODE 1)
+// Each nbar is written as Y = P nbar (the projection part of the RHS adjoint).
+template <class R>
+__device__ inline void adj_factors(const Geom& g, int64_t i, const AdjParams<R>& p, R& kx, R& ky, R& kz, R& E, R& E2,
+                                   bool& keep) {
+  int x, y, z, ikx, iky, ikz;
+  ks_of(g, i, x, y, z, ikx, iky, ikz);
+  kx = (R)ikx;
+  ky = (R)iky;
+  kz = (R)ikz;
+  const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
+  const R k2 = ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2;
+  E = exp_r(-p.nu * k2 * p.dt / (R)2);
+  E2 = exp_r(-p.nu * k2 * p.dt);
+  keep = k2 < p.kc2;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_adj_init(const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ vb,
+                                                  cpx<R>* __restrict__ Y, int64_t fs, Geom g, AdjParams<R> p) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  R kx, ky, kz, E, E2;
+  bool keep;
+  adj_factors(g, i, p, kx, ky, kz, E, E2, keep);
+  cpx<R> n[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const cpx<R> gq = keep ? E2 * lam[i + q * fs] : mk<R>(0, 0);
+    vb[i + q * fs] = gq;
+    n[q] = (p.dt / ((R)6 * E2)) * gq;
+  }
+  project(n, kx, ky, kz);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) Y[i + q * fs] = n[q];
+}
+
+// Spectral end of J^T: sbar = w (ubar^ + i k x wbar^ + S^T Sbar^) with the
+// c2r-adjoint weight w (2 interior, 1 on kx = 0 and kx = nx/2), then the
+// stage bookkeeping of MODE (0: vbar = sbar only).
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void k_adj_combine(const cpx<R>* __restrict__ W, int64_t fs,
+                                                     const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ vb,
+                                                     cpx<R>* __restrict__ Y, Geom g, AdjLayout L, AdjParams<R> p) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  R kx, ky, kz, E, E2;
+  bool keep;
+  adj_factors(g, i, p, kx, ky, kz, E, E2, keep);
+  const int x = (int)(i < g.nmain ? i % g.mx : g.mx);
+  const R wt = (x == 0 || x == g.mx) ? (R)1 : (R)2;
+  const cpx<R> o0 = W[i + 3 * fs], o1 = W[i + 4 * fs], o2 = W[i + 5 * fs];
+  cpx<R> sb[3];
+  sb[0] = W[i] + mul_i(ky * o2 - kz * o1);
+  sb[1] = W[i + fs] + mul_i(kz * o0 - kx * o2);
+  sb[2] = W[i + 2 * fs] + mul_i(kx * o1 - ky * o0);
+  if (L.les) {
+    // adjoint of S_ij = (i/2)(k_i u_j + k_j u_i): multiplication by -i/2 k
+    const cpx<R> s11 = W[i + 6 * fs], s12 = W[i + 7 * fs], s13 = W[i + 8 * fs];
+    const cpx<R> s22 = W[i + 9 * fs], s23 = W[i + 10 * fs], s33 = W[i + 11 * fs];
+    const R h = (R)0.5;
+    sb[0] = sb[0] - mul_i(kx * s11 + h * (ky * s12 + kz * s13));
+    sb[1] = sb[1] - mul_i(ky * s22 + h * (kx * s12 + kz * s23));
+    sb[2] = sb[2] - mul_i(kz * s33 + h * (kx * s13 + ky * s23));
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) sb[q] = wt * sb[q];
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) vb[i + q * fs] = sb[q];
+  } else if constexpr (MODE == 1) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) vb[i + q * fs] = vb[i + q * fs] + sb[q];
+  } else {
+    cpx<R> n[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const cpx<R> gq = keep ? E2 * lam[i + q * fs] : mk<R>(0, 0);
+      if constexpr (MODE == 4) {
+        vb[i + q * fs] = vb[i + q * fs] + E2 * sb[q];
+        n[q] = (p.dt / E) * ((R)(1.0 / 3.0) * gq + E2 * sb[q]);
+      } else if constexpr (MODE == 3) {
+        vb[i + q * fs] = vb[i + q * fs] + E * sb[q];
+        n[q] = (p.dt / E) * ((R)(1.0 / 3.0) * gq + ((R)0.5 * E) * sb[q]);
+      } else {
+        vb[i + q * fs] = vb[i + q * fs] + E * sb[q];
+        n[q] = p.dt * ((R)(1.0 / 6.0) * gq + ((R)0.5 * E) * sb[q]);
+      }
+    }
+    project(n, kx, ky, kz);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Y[i + q * fs] = n[q];
+  }
+}
+
+// Transpose of the kx = 0 Hermitian copy (symmetrize.f90:51-61) on Y:
+// for each target t = (0, y, z) (kz < 0, or kz = 0 and ky < 0) with source
+// s = (0, -y, -z): Y[s] += conj(Y[t]), Y[t] = 0.  Sources are never targets.
+template <class R>
+__global__ __launch_bounds__(256) void k_adj_hermT(cpx<R>* __restrict__ Y, int64_t fs, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n1 = (int64_t)(g.nz / 2 - 1) * g.ny;
+  const int64_t n2 = g.ny / 2 - 1;
+  int y, z, yp, zp;
+  if (i < n1) {
+    z = g.nz / 2 + 1 + (int)(i / g.ny);
+    y = (int)(i % g.ny);
+    zp = g.nz - z;
+    yp = (g.ny - y) % g.ny;
+  } else if (i < n1 + n2) {
+    z = 0;
+    y = g.ny / 2 + 1 + (int)(i - n1);
+    zp = 0;
+    yp = g.ny - y;
+  } else {
+    return;
+  }
+  const int64_t od = ((int64_t)z * g.ny + y) * g.mx, os = ((int64_t)zp * g.ny + yp) * g.mx;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const cpx<R> t = Y[od + q * fs];
+    Y[os + q * fs] = Y[os + q * fs] + conj(t);
+    Y[od + q * fs] = mk<R>(0, 0);
+  }
+}
+
+}  // namespace scl

@@ -12,6 +12,7 @@
 #include "ic.hpp"
 #include "ops.hpp"
 #include "ops_ext.hpp"
+#include "solver.hpp"
 
 namespace scl {
 
@@ -509,28 +510,6 @@ static RhsParams<R> rhs_params(const scl_plan3d* p, double nu, double dt, int fo
   return rp;
 }
 
-// Solver fields: V(nc) DU(nc) S(nc) W(...), nc = 3 velocity components
-// (+1 scalar), each fs entries; coef: forcing {a, b, E_f, Omega_f, uw} and
-// Smagorinsky {cs2, mm, lm} (8 doubles on the device).
-struct DevFields {
-  void* base;
-  int64_t fs;
-  int nc = 3;
-  double* coef = nullptr;
-  template <class R>
-  cpx<R>* f(int i) const {
-    return reinterpret_cast<cpx<R>*>(base) + i * fs;
-  }
-  template <class R>
-  cpx<R>* V() const { return f<R>(0); }
-  template <class R>
-  cpx<R>* DU() const { return f<R>(nc); }
-  template <class R>
-  cpx<R>* S() const { return f<R>(2 * nc); }
-  template <class R>
-  cpx<R>* W() const { return f<R>(3 * nc); }
-};
-
 // Event hooks for per-kernel timing (nullptr: none).
 struct Timer {
   std::vector<hipEvent_t>* ev = nullptr;
@@ -643,6 +622,17 @@ static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const s
   tm->mark(-1);
   return SCL_OK;
 }
+
+template <class R>
+int32_t scl::solver_stage(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st) {
+  if (prm.sgs || prm.scalar) return rk_stage_ext<R>(p, d, mode, prm, st, nullptr);
+  const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  return rk_stage<R>(p, d, mode, rp, prm.eps, prm.eta, prm.fkmax, st, nullptr);
+}
+template int32_t scl::solver_stage<float>(scl_plan3d*, const DevFields&, int, const scl_dns_params&,
+                                          const cpx<float>*);
+template int32_t scl::solver_stage<double>(scl_plan3d*, const DevFields&, int, const scl_dns_params&,
+                                           const cpx<double>*);
 
 extern "C" int32_t scl_ratesdudt(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* nx_, void* ny_,
                                  void* nz_, int32_t forced, double fkmax, double eps, double eta) {

@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void k_dyn_dots(const cpx<R>* __restrict__ Lf,
 }
 
 // out[j] = sum over blocks b of part[2b + j], j = 0, 1, in block order
-__global__ void k_sum_pairs(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+static __global__ void k_sum_pairs(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
   const int j = threadIdx.x;
   if (j >= 2) return;
   double s = 0;
@@ -391,7 +391,7 @@ __global__ void k_sum_pairs(const double* __restrict__ part, int nblocks, double
 }
 
 // cs2 = lm / mm from the summed partials, into coef[5..7] = {cs2, mm, lm}
-__global__ void k_dyn_cs2(const double* __restrict__ sums, double* __restrict__ coef) {
+static __global__ void k_dyn_cs2(const double* __restrict__ sums, double* __restrict__ coef) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     coef[6] = sums[0];
     coef[7] = sums[1];

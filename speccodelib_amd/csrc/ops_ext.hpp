@@ -1,5 +1,4 @@
-// Launchers of the LES + scalar RHS and the dynamic Smagorinsky update
-// (included by capi.hip only: its non-template kernels are defined here).
+// Launchers of the LES + scalar RHS and the dynamic Smagorinsky update.
 #pragma once
 
 #include "les.hpp"
