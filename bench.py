@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -75,6 +77,91 @@ def cpu_baseline(n, steps, scl):
     }
 
 
+# Uneven 2D vorticity ensembles: 192 x 256^2, 48 x 512^2, 16 x 1024^2, FP32,
+# nu = 1e-5 k^8 hyperviscosity, linear damping 0.1, forcing on 4 <= |k| <= 5,
+# dt = 5e-4, 1000 IF-RK4 steps.  Per RHS 15 F2 (y-inverse 1 in + 3 out,
+# x-pass 3 + 2, y-forward 2 + state and stage I/O), 58 F2 per step with
+# F2 = 8 (n/2 + 1) n bytes per member.
+ENS2D_GROUPS = [(256, 192), (512, 48), (1024, 16)]
+ENS2D_KW = dict(dt=5e-4, c_hyper=1e-5, n_hyper=8, damp=0.1, fkmin=4.0, fkmax=5.0, eps=0.01)
+ENS2D_STEP_F2 = 58.0
+
+
+def bench_ens2d(scl, steps, member_steps):
+    f2 = sum(m * 8 * (n // 2 + 1) * n for n, m in ENS2D_GROUPS)
+    eb = scl.Ensemble2D(ENS2D_GROUPS, prec=scl.SP, **ENS2D_KW)
+    eb.initialize(seed0=-1, k0=8.0, energy=0.5)
+    best = eb.tune(reps=3)
+    sched = [eb.get_schedule(g) for g in range(len(ENS2D_GROUPS))]
+    eb.step(5)
+    ms = eb.step_timed(steps)
+    d = eb.diagnostics()
+    eb.close()
+    ep = scl.Ensemble2D(ENS2D_GROUPS, prec=scl.SP, per_member=True, **ENS2D_KW)
+    ep.initialize(seed0=-1, k0=8.0, energy=0.5)
+    for g, sc in enumerate(sched):
+        ep.set_schedule(g, *sc)
+    ep.step(2)
+    msp = ep.step_timed(member_steps)
+    ep.close()
+    bms, pms = ms / steps, msp / member_steps
+    step_bytes = ENS2D_STEP_F2 * f2
+    finite = bool(np.all(np.isfinite(d)))
+    print(f"[bench] ens2d: batched {bms:.3f} ms/step, per-member {pms:.3f} ms/step", file=sys.stderr, flush=True)
+    return {
+        "workload": "2D vorticity ensembles 192x256^2 + 48x512^2 + 16x1024^2 FP32, IF-RK4 dt=5e-4, "
+                    "1e-5 k^8 hyperviscosity, damping 0.1, forcing 4<=|k|<=5 eps=0.01, IC k0=8 energy 0.5",
+        "batched_steps": steps,
+        "batched_ms_per_step": bms,
+        "batched_steps_per_s": 1e3 / bms,
+        "per_member_steps": member_steps,
+        "per_member_ms_per_step": pms,
+        "speedup_vs_per_member": pms / bms,
+        "step_alg_bytes": step_bytes,
+        "hbm_roofline_frac_step": step_bytes / (bms * 1e-3) / HBM_PEAK,
+        "ns_per_point_step": bms * 1e6 / sum(m * n * n for n, m in ENS2D_GROUPS),
+        "schedules_e_cw_rowgrid": sched,
+        "tuned_ms_per_rhs": [float(x) for x in best],
+        "finite": finite,
+        "energy_mean": float(d[:, 0].mean()),
+    }
+
+
+# Discrete adjoint: 128^3 static Smagorinsky LES (C_s = 0.15, nu = 0.005,
+# dt = 0.002), 200 IF-RK4 steps with checkpoints every 20, loss
+# sum' h (|u(T) - u_tgt|^2 + k^2 |u(T)|^2) (h = 1/2 on kx = 0), target from
+# seed -2 evolved the same way.  ratio = backward time (recomputation
+# included) / forward time.
+def bench_adjoint(scl, n=128, nsteps=200, ckpt=20, cs=0.15):
+    p = scl.Plan3D(n, n, n, scl.SP)
+    a = scl.Adjoint(p, nsteps, ckpt, nu=0.005, dt=0.002, sgs=1)
+    u0 = scl.initialize(n, n, n, -1, 3.0, 1.0, scl.SP)
+    tgt = a.forward(scl.initialize(n, n, n, -2, 3.0, 1.0, scl.SP), cs * cs)
+    uT = a.forward(u0, cs * cs)
+    kx, ky, kz = (np.arange(n // 2 + 1), np.fft.fftfreq(n, 1.0 / n), np.fft.fftfreq(n, 1.0 / n))
+    k2 = kx[None, None, :] ** 2 + ky[None, :, None] ** 2 + kz[:, None, None] ** 2
+    h = np.ones(k2.shape)
+    h[:, :, 0] = 0.5
+    gT = [(2 * h * (u - t) + 2 * h * k2 * u).astype(np.complex64) for u, t in zip(uT, tgt)]
+    loss = float(sum((h * (np.abs(u - t) ** 2 + k2 * np.abs(u) ** 2)).sum() for u, t in zip(uT, tgt)))
+    g0, dcs2 = a.backward(gT)
+    f, b = a.timing()
+    a.close()
+    p.close()
+    print(f"[bench] adjoint {n}^3: forward {f:.1f} ms, backward {b:.1f} ms, ratio {b / f:.2f}", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": f"adjoint of {nsteps} IF-RK4 steps, {n}^3 FP32 static Smagorinsky LES C_s={cs}, nu=0.005, "
+                    f"dt=0.002, checkpoints every {ckpt}",
+        "forward_ms": f,
+        "backward_ms": b,
+        "ratio": b / f,
+        "loss": loss,
+        "dL_dCs": 2 * cs * dcs2,
+        "grad_norm": float(np.sqrt(sum((np.abs(x) ** 2).sum() for x in g0))),
+    }
+
+
 def load_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
     if os.path.exists(path):
@@ -92,10 +179,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph-check", action="store_true")
+    ap.add_argument("--ens2d-steps", type=int, default=1000)
+    ap.add_argument("--ens2d-member-steps", type=int, default=20)
+    ap.add_argument("--no-ens2d", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path's multi-GPU sharding "
-              "is a design, not a build: see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding of this "
+              "path is a design, not a build", file=sys.stderr)
         sys.exit(2)
 
     import speccodelib_amd as scl
@@ -166,6 +257,10 @@ def main():
         sys.exit(3)
     del dns
     plan.close()
+    out["ens2d"] = None if a.no_ens2d else bench_ens2d(scl, a.ens2d_steps, a.ens2d_member_steps)
+    if not a.no_adjoint:
+        out["adjoint"] = bench_adjoint(scl)
+        out["adjoint_forward_ratio"] = out["adjoint"]["ratio"]
     if not a.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_steps, scl)

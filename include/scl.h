@@ -214,6 +214,37 @@ enum {
 int32_t scl_dns_step_timed(scl_dns *d, int32_t nsteps, double *kind_ms, int32_t *kind_count, double *total_ms);
 
 /* ---------------------------------------------------------------------
+ * RHS-callback form (rk5.f90:1-12: derivs(x, y, dydx, ny); the driver owns
+ * the stepping, derivs the physics).
+ * ------------------------------------------------------------------- */
+/* dydt = N(y) - nu k^2 y (scalar: nu/Sc) for the solver's configuration:
+ * y and dydt hold the solver's nc fields (3 velocity [+ scalar]) in the
+ * reference layout, on the host (on_device = 0) or the device.  t is unused
+ * (the system is autonomous). */
+int32_t scl_dns_rhs(scl_dns *d, double t, const void *const *y, void *const *dydt, int32_t on_device);
+/* The solver's state on the device: nc fields of the split layout, *n
+ * complex entries in all (the layout of scl_dns_derivs_cb's arrays). */
+int32_t scl_dns_state(scl_dns *d, void **state, int64_t *n);
+/* nu k^2 (scalar: nu/Sc k^2) per state entry into a device array of *n reals. */
+int32_t scl_dns_rates(scl_dns *d, void *rate);
+/* The callback type of scl_ifrk4_integrate: dydt = f(t, y) on device arrays;
+ * returns after dydt is complete. */
+typedef int32_t (*scl_derivs_fn)(void *ctx, double t, const void *y, void *dydt);
+/* scl_dns_rhs's N(y) without the linear term, as a callback: ctx = scl_dns*,
+ * arrays in the split layout of scl_dns_state. */
+int32_t scl_dns_derivs_cb(void *ctx, double t, const void *y, void *dydt);
+/* Lawson IF-RK4 (dns-deform.f90:160-237's scheme) of y' = -rate y + f(t, y)
+ * over n complex device entries: y in place, rate a device array of n reals
+ * in the precision prec (NULL: classical RK4), f called four times per step. */
+int32_t scl_ifrk4_integrate(int32_t prec, int64_t n, void *y, const void *rate, double t0, double dt,
+                            int32_t nsteps, scl_derivs_fn f, void *ctx);
+/* derivs(x, y, dydx, ny) for Fortran callers (external scl_derivs): y and
+ * dydx are the bound FP64 solver's nc fields as real(dp), ny reals;
+ * failures abort, as the reference STOPs. */
+int32_t scl_derivs_bind(scl_dns *d);
+void scl_derivs_(const double *x, const double *y, double *dydx, const int32_t *ny);
+
+/* ---------------------------------------------------------------------
  * 2D vorticity ensembles.  Replaces the time loop of spectral_dns_2d.f90
  * (:94 integrating factor, :161-188 RHS) for many independent members of
  * several grid sizes at once.  A member's field is the reference's

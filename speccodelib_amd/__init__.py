@@ -156,6 +156,13 @@ def lib():
             "scl_ens2d_set_schedule": ([P, I32, I32, I32, I32], I32),
             "scl_ens2d_get_schedule": ([P, I32, P, P, P], I32),
             "scl_ens2d_tune": ([P, I32, P], I32),
+            "scl_dns_rhs": ([P, D, P, P, I32], I32),
+            "scl_dns_state": ([P, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int64)], I32),
+            "scl_dns_rates": ([P, P], I32),
+            "scl_dns_derivs_cb": ([P, D, P, P], I32),
+            "scl_ifrk4_integrate": ([I32, ctypes.c_int64, P, P, D, D, I32, P, P], I32),
+            "scl_derivs_bind": ([P], I32),
+            "scl_derivs_": ([P, P, P, P], None),
             "scl_adjoint_create": ([P, ctypes.POINTER(ScDnsParams), I32, I32, ctypes.POINTER(P)], I32),
             "scl_adjoint_destroy": ([P], I32),
             "scl_adjoint_forward": ([P, D] + [P] * 6 + [I32], I32),
@@ -394,6 +401,27 @@ class DNS:
         _check(lib().scl_dns_diagnostics(self.h, ctypes.byref(d)))
         return {k: getattr(d, k) for k, _ in ScDiag._fields_}
 
+    def rhs(self, *y):
+        """dydt = N(y) - nu k^2 y (the derivs form) for host fields y."""
+        nc = len(y)
+        for a in y:
+            _field(a, self.plan.prec, self.plan.shape)
+        out = [self.plan.zeros() for _ in range(nc)]
+        yi = (ctypes.c_void_p * nc)(*[_ptr(a) for a in y])
+        yo = (ctypes.c_void_p * nc)(*[_ptr(a) for a in out])
+        _check(lib().scl_dns_rhs(self.h, 0.0, yi, yo, 0))
+        return tuple(out)
+
+    def state(self):
+        """(device pointer, complex entries) of the state in the split layout."""
+        ptr = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        _check(lib().scl_dns_state(self.h, ctypes.byref(ptr), ctypes.byref(n)))
+        return ptr.value, n.value
+
+    def rates(self, rate_ptr):
+        _check(lib().scl_dns_rates(self.h, rate_ptr))
+
     def step_timed(self, n):
         """Runs n steps with HIP events around every kernel launch; returns
         (total_ms, {kind: (ms, launches)})."""
@@ -552,3 +580,14 @@ class Adjoint:
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().scl_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+
+def ifrk4_integrate(prec, n, y_ptr, rate_ptr, t0, dt, nsteps, fn_ptr, ctx):
+    """Device Lawson IF-RK4 of y' = -rate y + f(t, y) with a C callback f
+    (``fn_ptr``: address of an ``scl_derivs_fn``)."""
+    _check(lib().scl_ifrk4_integrate(prec, n, y_ptr, rate_ptr, t0, dt, nsteps, fn_ptr, ctx))
+
+
+def dns_derivs_cb_ptr():
+    """Address of the built-in spectral RHS callback (ctx = a DNS handle)."""
+    return ctypes.cast(lib().scl_dns_derivs_cb, ctypes.c_void_p).value

@@ -11,9 +11,17 @@ is synchronised around each call, since the library runs on its own stream.
 """
 from __future__ import annotations
 
-import torch
+import sys
 
-from . import Adjoint
+import speccodelib_amd as _scl
+
+if _scl._lib is not None and "torch" not in sys.modules:
+    raise RuntimeError("import torch before libscl.so is loaded: torch bundles its own HIP runtime under the "
+                       "same soname, and a process that loads libscl.so first cannot give torch the GPU")
+
+import torch  # noqa: E402
+
+from . import Adjoint  # noqa: E402
 
 
 def _ptrs(t):

@@ -3,6 +3,16 @@ import sys
 
 import pytest
 
+# torch ships its own HIP runtime under the same soname as the system one
+# that libscl.so links.  Importing torch before libscl.so is loaded makes the
+# process use a single runtime (torch's), which libscl.so then binds to;
+# loading libscl.so first would leave torch with a second runtime that
+# cannot see the GPU.
+try:
+    import torch  # noqa: F401
+except Exception:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
