@@ -184,6 +184,7 @@ typedef struct scl_diag {
   double scalar_variance;    /* sum' |phi|^2 (= <phi^2>/2) */
   double scalar_dissipation; /* 2 (nu/Sc) sum' k^2 |phi|^2 */
   double cs2;                /* current Smagorinsky coefficient (sgs > 0) */
+  double helicity_dissipation; /* 2 nu sum' k^2 2 Re(w . conj v) */
 } scl_diag;
 
 int32_t scl_dns_create(scl_plan3d *plan, const scl_dns_params *p, scl_dns **out);

@@ -69,6 +69,7 @@ class ScDiag(ctypes.Structure):
         ("scalar_variance", ctypes.c_double),
         ("scalar_dissipation", ctypes.c_double),
         ("cs2", ctypes.c_double),
+        ("helicity_dissipation", ctypes.c_double),
     ]
 
 

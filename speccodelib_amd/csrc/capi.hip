@@ -932,7 +932,7 @@ static int32_t diag_run(scl_plan3d* p, const cpx<R>* v, int nbins, std::vector<d
   SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * len * sizeof(double) + 64, &buf));
   double* part = (double*)buf;
   double* fin = part + (size_t)DIAG_BLOCKS * len;
-  const size_t shbytes = (size_t)(4 * 2 * nbins + 256 * 5) * sizeof(double);
+  const size_t shbytes = (size_t)(4 * 2 * nbins + 256 * DIAG_NT) * sizeof(double);
   hipLaunchKernelGGL((k_diag_partial<R>), dim3(DIAG_BLOCKS), dim3(256), shbytes, p->stream, v, p->fstride(), g, nbins,
                      part);
   SCL_CHECK(check_launch("diag_partial"));
@@ -1024,6 +1024,7 @@ extern "C" int32_t scl_dns_diagnostics(scl_dns* d, scl_diag* out) {
     out->enstrophy = r[2 * nb + 1];
     out->dissipation = 2.0 * d->p.nu * r[2 * nb + 1];
     out->helicity = r[2 * nb + 2];
+    out->helicity_dissipation = 2.0 * d->p.nu * r[2 * nb + 5];
     out->divrms_spec = std::sqrt(r[2 * nb + 3]);
     out->nonfinite = r[2 * nb + 4] > 0 ? 1 : 0;
     out->scalar_variance = 0;

@@ -8,19 +8,20 @@
 
 namespace scl {
 
-constexpr int DIAG_TOT = 8;  // E, 2nu-free dissipation sum k2 E, H, |k.v|^2, nonfinite, spare...
+constexpr int DIAG_TOT = 8;  // E, sum k2 E, H, |k.v|^2, nonfinite, sum k2 H, spare
+constexpr int DIAG_NT = 6;   // totals accumulated per thread
 constexpr int DIAG_BLOCKS = 512;
 
 template <class R>
 __global__ __launch_bounds__(256) void k_diag_partial(const cpx<R>* __restrict__ v, int64_t fstride, Geom g, int nbins,
                                                       double* __restrict__ part) {
-  extern __shared__ double sh[];  // 4 waves x (2 nbins) + 256 x 4 totals
+  extern __shared__ double sh[];  // 4 waves x (2 nbins) + 256 x DIAG_NT totals
   const int tid = threadIdx.x, wave = tid / 64;
   for (int i = tid; i < 4 * 2 * nbins; i += 256) sh[i] = 0.0;
   __syncthreads();
   double* hE = sh + wave * 2 * nbins;
   double* hH = hE + nbins;
-  double tE = 0, tK = 0, tH = 0, tD = 0, tN = 0;
+  double tE = 0, tK = 0, tH = 0, tD = 0, tN = 0, tHK = 0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + tid; i < g.ntot; i += (int64_t)gridDim.x * 256) {
     int x, y, z;
     split_xyz(g, i, x, y, z);
@@ -39,6 +40,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(const cpx<R>* __restrict__
     tE += e;
     tK += k2 * e;
     tH += h;
+    tHK += k2 * h;
     tD += w * (dr * dr + di * di);
     if (!(isfinite(ar) && isfinite(ai) && isfinite(br) && isfinite(bi) && isfinite(cr) && isfinite(ci))) tN += 1.0;
     const int ll = (int)floor(sqrt(k2) + 0.5);
@@ -48,11 +50,12 @@ __global__ __launch_bounds__(256) void k_diag_partial(const cpx<R>* __restrict__
     }
   }
   double* tot = sh + 4 * 2 * nbins;
-  tot[tid * 5 + 0] = tE;
-  tot[tid * 5 + 1] = tK;
-  tot[tid * 5 + 2] = tH;
-  tot[tid * 5 + 3] = tD;
-  tot[tid * 5 + 4] = tN;
+  tot[tid * DIAG_NT + 0] = tE;
+  tot[tid * DIAG_NT + 1] = tK;
+  tot[tid * DIAG_NT + 2] = tH;
+  tot[tid * DIAG_NT + 3] = tD;
+  tot[tid * DIAG_NT + 4] = tN;
+  tot[tid * DIAG_NT + 5] = tHK;
   __syncthreads();
   const int stride = 2 * nbins + DIAG_TOT;
   double* out = part + (int64_t)blockIdx.x * stride;
@@ -61,9 +64,9 @@ __global__ __launch_bounds__(256) void k_diag_partial(const cpx<R>* __restrict__
     for (int wv = 0; wv < 4; ++wv) s += sh[wv * 2 * nbins + j];
     out[j] = s;
   }
-  if (tid < 5) {
+  if (tid < DIAG_NT) {
     double s = 0;
-    for (int t = 0; t < 256; ++t) s += tot[t * 5 + tid];
+    for (int t = 0; t < 256; ++t) s += tot[t * DIAG_NT + tid];
     out[2 * nbins + tid] = s;
   }
 }

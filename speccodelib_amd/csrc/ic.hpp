@@ -104,12 +104,24 @@ int32_t initialize3d(R* vx, R* vy, R* vz, int nx, int ny, int nz, int32_t iseed,
   const R tiny = std::numeric_limits<R>::min();
   Ran1 ran(iseed);
   R* v[3] = {vx, vy, vz};
+  const R kcut = kcut_d > 0 ? (R)kcut_d : (R)(2.0f * (float)(nx / 2) / 3.0f);
+  const R kcut2 = kcut * kcut;
   for (int kk = 0; kk < lz; ++kk) {
     const R kz = (R)kwave(kk, lz);
     for (int jj = 0; jj < ly; ++jj) {
       const R ky = (R)kwave(jj, ly);
       for (int ii = 0; ii < lx1; ++ii) {
         const R kx = (R)ii;
+        const int ik2 = ii * ii + kwave(jj, ly) * kwave(jj, ly) + kwave(kk, lz) * kwave(kk, lz);
+        const int64_t o = 2 * (((int64_t)kk * ly + jj) * lx1 + ii);
+        if (ik2 > 0 && (R)ik2 >= kcut2) {
+          // the 2/3 cut below zeroes this mode (and its kx = 0 mirror, at
+          // the same |k|): draw its six numbers to keep the stream, skip
+          // the Gaussians
+          for (int q = 0; q < 6; ++q) ran.next();
+          for (int q = 0; q < 3; ++q) v[q][o] = v[q][o + 1] = 0;
+          continue;
+        }
         R g[3][2];
         for (int q = 0; q < 3; ++q) {
           const R t1 = ran.uniform<R>();
@@ -121,7 +133,6 @@ int32_t initialize3d(R* vx, R* vy, R* vz, int nx, int ny, int nz, int32_t iseed,
         const R tr = (kx * g[0][0] + ky * g[1][0] + kz * g[2][0]) / den;
         const R ti = (kx * g[0][1] + ky * g[1][1] + kz * g[2][1]) / den;
         const R k[3] = {kx, ky, kz};
-        const int64_t o = 2 * (((int64_t)kk * ly + jj) * lx1 + ii);
         for (int q = 0; q < 3; ++q) {
           v[q][o] = g[q][0] - k[q] * tr;
           v[q][o + 1] = g[q][1] - k[q] * ti;
@@ -130,8 +141,6 @@ int32_t initialize3d(R* vx, R* vy, R* vz, int nx, int ny, int nz, int32_t iseed,
     }
   }
   for (int q = 0; q < 3; ++q) symmetrize_host<R>(v[q], lx1, ly, lz);
-  const R kcut = kcut_d > 0 ? (R)kcut_d : (R)(2.0f * (float)(nx / 2) / 3.0f);
-  const R kcut2 = kcut * kcut;
   const R akp = (R)akp_d, u0 = (R)u0_d;
   const R pref = u0 * std::sqrt((R)8 * std::sqrt((R)2 / pi) / ((R)3 * pi * akp * akp * akp * akp * akp));
   double esum = 0.0;

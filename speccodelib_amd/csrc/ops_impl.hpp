@@ -95,10 +95,12 @@ int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int
   });
 }
 
-// Column count of the fused z-pass kernels for a grid.
+// Column count of the fused z-pass kernels for a grid.  1024-point lines
+// stay at 8 columns: 16 would make 1024-thread workgroups, whose 128-VGPR
+// cap spills these kernels.
 template <class R>
 static bool zcol16(const Geom& g) {
-  return sizeof(R) == 4 && g.mx % 16 == 0 && g.ny % 16 == 0;
+  return sizeof(R) == 4 && g.mx % 16 == 0 && g.ny % 16 == 0 && g.nz <= 512;
 }
 
 template <class R>

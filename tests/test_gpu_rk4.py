@@ -160,3 +160,13 @@ def test_diagnostics_match_oracle_fp64():
     assert np.isclose(dg["dissipation"], orc.dissipation(*v, k2, 0.01), rtol=1e-13)
     assert np.isclose(dg["enstrophy"], orc.enstrophy_total(*v, k2), rtol=1e-13)
     assert dg["divmax"] < 1e-12
+    # helicity sum' 2 Re(w . v*) and its dissipation 2 nu sum' k^2 (...): the
+    # shell spectrum above sums to the first
+    w = orc.vorticity(*v, kx, ky, kz)
+    h = 2 * np.real(sum(a * np.conj(b) for a, b in zip(w, v)))
+    hw = np.ones(k2.shape)
+    hw[:, :, 0] = 0.5
+    k2i = k2.astype(np.float64)
+    k2i[0, 0, 0] = 0
+    assert np.isclose(dg["helicity"], (hw * h).sum(), rtol=1e-12)
+    assert np.isclose(dg["helicity_dissipation"], 2 * 0.01 * (hw * k2i * h).sum(), rtol=1e-12)
