@@ -31,14 +31,12 @@ def test_initialize_matches_oracle(prec, rt):
             assert relerr(a, b) < rt
 
 
-@pytest.mark.parametrize("forced", [0, 1, 2])
+@pytest.mark.parametrize("forced", [0, 1])
 def test_rk4step_fp64_matches_oracle(forced):
     n = 32
     print(f"case rk4step dp forced={forced}", flush=True)
     p = scl.Plan3D(n, n, n, scl.DP)
     v = [a.copy() for a in orc.initialize(n, n, n, 1, 4.0, 1.0)]
-    if forced == 2:
-        pytest.skip("helical stepping is covered by the device-solver test below")
     d = orc.DNS(orc.DNSParams(nx=n, nu=0.01, dt=0.002, forced=bool(forced), fkmax=2.0, eps=0.1))
     w = tuple(v)
     for _ in range(3):
