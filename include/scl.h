@@ -224,7 +224,8 @@ int32_t scl_dns_step_timed(scl_dns *d, int32_t nsteps, double *kind_ms, int32_t 
  * (the system is autonomous). */
 int32_t scl_dns_rhs(scl_dns *d, double t, const void *const *y, void *const *dydt, int32_t on_device);
 /* The solver's state on the device: nc fields of the split layout, *n
- * complex entries in all (the layout of scl_dns_derivs_cb's arrays). */
+ * complex entries in all (the layout of scl_dns_derivs_cb's arrays).  The
+ * caller may change the state through the pointer before the next step. */
 int32_t scl_dns_state(scl_dns *d, void **state, int64_t *n);
 /* nu k^2 (scalar: nu/Sc k^2) per state entry into a device array of *n reals. */
 int32_t scl_dns_rates(scl_dns *d, void *rate);

@@ -116,10 +116,10 @@ struct LinIdx {
 #ifndef SCL_COL_CW
 #define SCL_COL_CW 8
 #endif
-template <class R, int N, int CWv = SCL_COL_CW>
+template <class R, int N, int CWv = SCL_COL_CW, int EV = 0>
 struct ColCfg {
   static constexpr int CW = CWv;
-  static constexpr int E = N <= 512 ? 8 : 16;
+  static constexpr int E = EV ? EV : (N <= 512 ? 8 : 16);
   static constexpr int T = N / E;
   static constexpr int BS = CW * T;
 };
@@ -159,64 +159,151 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
   for (int m = 0; m < E; ++m) f[t.base + (int64_t)Line<R, N, E, DIR>::out_pos(m, rt) * t.rs] = v[m];
 }
 
-// ZINV: z-inverse of u and of w = i k x u.  s: stage state (3 fields at
-// stride sstride); out: 6 fields (u then w) at stride wstride.
-template <class R, int N, int CWZ>
-__global__ __launch_bounds__((ColCfg<R, N, CWZ>::BS)) void k_zinv_vort(const cpx<R>* __restrict__ s, int64_t sstride,
-                                                                 cpx<R>* __restrict__ w, int64_t wstride, Geom g,
-                                                                 const cpx<R>* __restrict__ tw, int tws) {
-  using CC = ColCfg<R, N, CWZ>;
+// z-inverse of a velocity column held in registers (slot m = index
+// rt + T m along z, as every transform here leaves it) and of w_x, w_y of
+// w = i k x u (dns-deform.f90:268-270), by linearity of the z-transform Z:
+//   Z(wx) = i (ky Uz - Z(kz uy)),  Z(wy) = i (Z(kz ux) - kx Uz),  Uq = Z(uq):
+// five transforms, stored to w[0..4] (u, then w_x, w_y) at stride ws.
+// w_z = i (kx u_y - ky u_x) needs no kz, so the y-inverse builds it from
+// the transformed u_x, u_y (k_yinv_vort) and one field less crosses HBM.
+template <class R, int N, int E, int CW>
+__device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds,
+                                  const cpx<R>* twl, int rt, int c, R kx, R ky, const int64_t (&off)[E],
+                                  const R (&kz)[E], cpx<R>* __restrict__ w, int64_t ws) {
+  using L = Line<R, N, E, +1>;
+  cpx<R> d[E];
+  L::run(uz, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + 2 * ws] = uz[m];
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = kz[m] * uy[m];
+  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + 3 * ws] = mul_i(ky * uz[m] - d[m]);
+  L::run(uy, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + ws] = uy[m];
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = kz[m] * ux[m];
+  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m] + 4 * ws] = mul_i(d[m] - kx * uz[m]);
+  L::run(ux, lds, rt, twl, 1, ColIdx<CW>{c});
+#pragma unroll
+  for (int m = 0; m < E; ++m) w[off[m]] = ux[m];
+}
+
+// ZINV: z-inverse of the stage state s (3 fields at stride sstride) into
+// w[0..4] (stride wstride), see zinv_store.
+template <class R, int N, int CWZ, int EZ = 0>
+__global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const cpx<R>* __restrict__ s,
+                                                                     int64_t sstride, cpx<R>* __restrict__ w,
+                                                                     int64_t wstride, Geom g,
+                                                                     const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N, CWZ, EZ>;
   constexpr int CW = CC::CW, E = CC::E, T = CC::T;
   __shared__ cpx<R> lds[lds_pad(N) * CW];
   __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
   for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
-  const R kx = (R)t.kx, ky = (R)kwave(t.oth, g.ny);
-  using L = Line<R, N, E, +1>;
-  // Row of slot m, on entry (kz index) and after the transform (z index):
-  // the last Stockham pass leaves slot m at index rt + T*m.
   int64_t off[E];
   R kz[E];
+  cpx<R> ux[E], uy[E], uz[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     off[m] = t.base + (int64_t)(rt + T * m) * t.rs;
     kz[m] = (R)kwave(rt + T * m, g.nz);
-  }
-  // w = i k x u (dns-deform.f90:268-270), by linearity of the z-transform Z:
-  //   Z(wx) = i (ky Uz - Z(kz uy)),  Z(wy) = i (Z(kz ux) - kx Uz),
-  //   Z(wz) = i (kx Uy - ky Ux),     Uq = Z(uq): five transforms, not six.
-  cpx<R> uz[E], uy[E], ux[E], d[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) uz[m] = s[off[m] + 2 * sstride];
-  L::run(uz, lds, rt, twl, 1, ColIdx<CW>{c});
-#pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + 2 * wstride] = uz[m];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
+    uz[m] = s[off[m] + 2 * sstride];
     uy[m] = s[off[m] + sstride];
-    d[m] = kz[m] * uy[m];
-  }
-  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-#pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + 3 * wstride] = mul_i(ky * uz[m] - d[m]);
-  L::run(uy, lds, rt, twl, 1, ColIdx<CW>{c});
-#pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + wstride] = uy[m];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
     ux[m] = s[off[m]];
-    d[m] = kz[m] * ux[m];
   }
+  zinv_store<R, N, E, CW>(ux, uy, uz, lds, twl, rt, c, (R)t.kx, (R)kwave(t.oth, g.ny), off, kz, w, wstride);
+}
+
+// YINV: y-inverse of the five z-pass outputs plus w_z = i (kx Uy - ky Ux),
+// formed before the transform while ky is still a spectral index.  Reads 5
+// fields, writes 6, in place at stride fs.  blockIdx.y = 0 transforms u_x,
+// u_y and w_z; blockIdx.y = 1..3 transform fields 2..4 (u_z, w_x, w_y).
+template <class R, int N>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __restrict__ W, int64_t fs, Geom g,
+                                                                const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 1>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  using L = Line<R, N, E, +1>;
+  auto load = [&](cpx<R>(&v)[E], int f) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = t.ok ? W[t.base + (int64_t)(rt + T * m) * t.rs + f * fs] : mk<R>(0, 0);
+  };
+  auto store = [&](cpx<R>(&v)[E], int f) {
+    if (!t.ok) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) W[t.base + (int64_t)L::out_pos(m, rt) * t.rs + f * fs] = v[m];
+  };
+  cpx<R> a[E];
+  if (blockIdx.y > 0) {
+    const int f = 1 + (int)blockIdx.y;
+    load(a, f);
+    L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(a, f);
+    return;
+  }
+  const R kx = (R)t.kx;
+  cpx<R> b[E], d[E];
+  load(a, 0);
+  load(b, 1);
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(kx * b[m] - (R)kwave(rt + T * m, g.ny) * a[m]);
   L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-#pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + 4 * wstride] = mul_i(d[m] - kx * uz[m]);
-  L::run(ux, lds, rt, twl, 1, ColIdx<CW>{c});
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    w[off[m]] = ux[m];
-    w[off[m] + 5 * wstride] = mul_i(kx * uy[m] - ky * ux[m]);
+  store(d, 5);
+  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(a, 0);
+  L::run(b, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(b, 1);
+}
+
+// One IF-RK4 stage of one component in the division-free form (see
+// k_zfwd_rhs): a = dt N, em1 = expm1(x), x = -rate dt/2, so E y = y + em1 y
+// and E^2 y = y + expm1(2x) y, each exact to the rounding of the result.
+// Returns the new stage state (MODE 1..3) or the new v (MODE 4); the stage
+// state is stored only where store_s.
+template <int MODE, class R>
+__device__ inline cpx<R> stage_keep(cpx<R> a, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
+                                    bool store_s) {
+  cpx<R> out;
+  if constexpr (MODE == 1) {
+    const cpx<R> v = V[o];
+    const cpx<R> u = v + ((R)1 / (R)6) * a, q = v + (R)0.5 * a;
+    DU[o] = u + expm1_r((R)2 * x) * u;
+    out = q + em1 * q;
+  } else if constexpr (MODE == 2 || MODE == 3) {
+    const cpx<R> v = V[o];
+    const cpx<R> q = ((R)1 / (R)3) * a;
+    DU[o] = DU[o] + (q + em1 * q);
+    const cpx<R> ev = v + em1 * v;
+    if constexpr (MODE == 2) {
+      out = ev + (R)0.5 * a;
+    } else {
+      const cpx<R> u = ev + a;
+      out = u + em1 * u;
+    }
+  } else {
+    out = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
+    V[o] = out;
+    return out;
   }
+  if (store_s) S[o] = out;
+  return out;
+}
+
+template <int MODE, class R>
+__device__ inline void stage_update(cpx<R> a, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
+  stage_keep<MODE, R>(a, o, V, DU, S, em1, x, cut, true);
 }
 
 // Parameters of the fused ZFWD epilogue.
@@ -234,17 +321,26 @@ struct RhsParams {
 // pointwise part of symmetrize + projection (+ optional RHS truncation),
 // then, for MODE 1..4, the IF-RK4 stage update; MODE 0 writes N(u) back
 // into W.  st is the stage state the forcing uses (v for stage 1).
-//   MODE 1 (dns-deform.f90:164-172, 177-179): k = dt N; du = k; S = E (v + k/2)
-//   MODE 2 (:181-196):  k = dt N / E; du += 2k; S = E (v + k/2)
-//   MODE 3 (:198-213):  k = dt N / E; du += 2k; S = E E (v + k)
-//   MODE 4 (:215-237):  k = dt N / E / E; du += k; v = E E (v + du/6); v = 0 where k2 >= kc2
-// with E = exp(-nu k2 dt / 2) (:90).
-template <class R, int N, int MODE, int CWZ>
-__global__ __launch_bounds__((ColCfg<R, N, CWZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V, cpx<R>* DU,
-                                                                cpx<R>* S, const cpx<R>* st,
-                                                                const double* __restrict__ coef, RhsParams<R> p, Geom g,
-                                                                const cpx<R>* __restrict__ tw, int tws) {
-  using CC = ColCfg<R, N, CWZ>;
+// The scheme of dns-deform.f90:164-237 (k1 = dt N(v), s2 = E (v + k1/2),
+// k2 = dt N(s2)/E, s3 = E (v + k2/2), k3 = dt N(s3)/E, s4 = E^2 (v + k3),
+// k4 = dt N(s4)/E^2, v' = E^2 (v + (k1 + 2 k2 + 2 k3 + k4)/6), cut at
+// kc2), E = exp(-nu k2 dt/2) (:90), is kept with the accumulator du holding
+// E^2 v + ... instead, a = dt N:
+//   MODE 1: du = E^2 (v + a/6);  S = E (v + a/2)
+//   MODE 2: du += E a/3;         S = E v + a/2
+//   MODE 3: du += E a/3;         S = E (E v + a)
+//   MODE 4: v = du + a/6, 0 where k2 >= kc2
+// so the last stage reads no v, and no stage divides by E.
+// FUSED (MODE 1..4): the new stage state (or v) stays in registers and its
+// z-inverse goes to W[0..4] for the next RHS (zinv_store), so that RHS
+// needs no z-inverse pass of its own; S is then stored only where the
+// forcing reads it (k2 < fk2).
+template <class R, int N, int MODE, int CWZ, int EZ = 0, bool FUSED = false>
+__global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V,
+                                                                    cpx<R>* DU, cpx<R>* S, const cpx<R>* st,
+                                                                    const double* __restrict__ coef, RhsParams<R> p,
+                                                                    Geom g, const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N, CWZ, EZ>;
   constexpr int CW = CC::CW, E = CC::E, T = CC::T;
   __shared__ cpx<R> lds[lds_pad(N) * CW];
   __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
@@ -323,38 +419,23 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ>::BS)) void k_zfwd_rhs(cpx<R>* W,
       // that compounds every step; E*w is applied as w + expm1(x)*w, exact
       // to the rounding of the result.
       const R x = -p.nu * k2 * p.dt / (R)2;
-      const R ef = exp_r(x);
-      const R em1 = expm1_r(x), em2 = expm1_r((R)2 * x);
-      const cpx<R> n3[3] = {a, b, d};
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int64_t oq = o + q * fstride;
-        const cpx<R> v = V[oq];
-        if constexpr (MODE == 1) {
-          const cpx<R> k = p.dt * n3[q];
-          DU[oq] = k;
-          const cpx<R> w = v + (R)0.5 * k;
-          S[oq] = w + em1 * w;
-        } else if constexpr (MODE == 2 || MODE == 3) {
-          const cpx<R> k = mk<R>(p.dt * n3[q].x / ef, p.dt * n3[q].y / ef);
-          DU[oq] = DU[oq] + (R)2 * k;
-          if constexpr (MODE == 2) {
-            const cpx<R> w = v + (R)0.5 * k;
-            S[oq] = w + em1 * w;
-          } else {
-            const cpx<R> w = v + k;
-            S[oq] = w + em2 * w;
-          }
-        } else {
-          const cpx<R> k = mk<R>(p.dt * n3[q].x / ef / ef, p.dt * n3[q].y / ef / ef);
-          const cpx<R> du = DU[oq] + k;
-          const cpx<R> w = mk<R>(v.x + du.x / (R)6, v.y + du.y / (R)6);
-          cpx<R> vn = w + em2 * w;
-          if (k2 >= p.kc2) vn = mk<R>(0, 0);
-          V[oq] = vn;
-        }
-      }
+      const R em1 = expm1_r(x);
+      const bool cut = k2 >= p.kc2;
+      const bool store_s = !FUSED || (p.forced && k2 < p.fk2);
+      nx_[m] = stage_keep<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut, store_s);
+      ny_[m] = stage_keep<MODE, R>(p.dt * b, o + fstride, V, DU, S, em1, x, cut, store_s);
+      nz_[m] = stage_keep<MODE, R>(p.dt * d, o + 2 * fstride, V, DU, S, em1, x, cut, store_s);
     }
+  }
+  if constexpr (FUSED && MODE > 0) {
+    int64_t off[E];
+    R kzs[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      off[m] = t.base + (int64_t)(rt + T * m) * t.rs;
+      kzs[m] = (R)kwave(rt + T * m, g.nz);
+    }
+    zinv_store<R, N, E, CW>(nx_, ny_, nz_, lds, twl, rt, c, kx, ky, off, kzs, W, fstride);
   }
 }
 

@@ -134,34 +134,6 @@ struct ExtParams {
   int sign_phi;  // +1: R = -i k.F (physical advection)
 };
 
-// One IF-RK4 stage update of one field component (see k_zfwd_rhs).
-template <int MODE, class R>
-__device__ inline void rk_update(cpx<R> n, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R ef, R em1, R em2,
-                                 bool cut) {
-  const cpx<R> v = V[o];
-  if constexpr (MODE == 1) {
-    const cpx<R> k = dt * n;
-    DU[o] = k;
-    const cpx<R> w = v + (R)0.5 * k;
-    S[o] = w + em1 * w;
-  } else if constexpr (MODE == 2 || MODE == 3) {
-    const cpx<R> k = mk<R>(dt * n.x / ef, dt * n.y / ef);
-    DU[o] = DU[o] + (R)2 * k;
-    if constexpr (MODE == 2) {
-      const cpx<R> w = v + (R)0.5 * k;
-      S[o] = w + em1 * w;
-    } else {
-      const cpx<R> w = v + k;
-      S[o] = w + em2 * w;
-    }
-  } else {
-    const cpx<R> k = mk<R>(dt * n.x / ef / ef, dt * n.y / ef / ef);
-    const cpx<R> du = DU[o] + k;
-    const cpx<R> w = mk<R>(v.x + du.x / (R)6, v.y + du.y / (R)6);
-    V[o] = cut ? mk<R>(0, 0) : w + em2 * w;
-  }
-}
-
 // Combine (spectral): N = u x w + f - i k_j tau_ij (spectral_les.f90:232-234),
 // symmetrize's zeroing, projection; R_phi = -i k . F - G u_z
 // (spectral_dns_scalar.f90:236,245), zeroing; then the IF-RK4 stage (MODE
@@ -233,14 +205,14 @@ __global__ __launch_bounds__(256) void k_ext_combine(cpx<R>* W, int64_t fs, cpx<
     if (L.scalar) W[i + 3 * fs] = rp;
   } else {
     const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
-    const R ef = exp_r(xv), em1 = expm1_r(xv), em2 = expm1_r((R)2 * xv);
+    const R em1 = expm1_r(xv);
     const bool cut = k2 >= p.r.kc2;
-    rk_update<MODE, R>(a, i, V, DU, S, p.r.dt, ef, em1, em2, cut);
-    rk_update<MODE, R>(b, i + fs, V, DU, S, p.r.dt, ef, em1, em2, cut);
-    rk_update<MODE, R>(d, i + 2 * fs, V, DU, S, p.r.dt, ef, em1, em2, cut);
+    stage_update<MODE, R>(p.r.dt * a, i, V, DU, S, em1, xv, cut);
+    stage_update<MODE, R>(p.r.dt * b, i + fs, V, DU, S, em1, xv, cut);
+    stage_update<MODE, R>(p.r.dt * d, i + 2 * fs, V, DU, S, em1, xv, cut);
     if (L.scalar) {
       const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
-      rk_update<MODE, R>(rp, i + 3 * fs, V, DU, S, p.r.dt, exp_r(xp), expm1_r(xp), expm1_r((R)2 * xp), cut);
+      stage_update<MODE, R>(p.r.dt * rp, i + 3 * fs, V, DU, S, expm1_r(xp), xp, cut);
     }
   }
 }

@@ -95,12 +95,38 @@ int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int
   });
 }
 
+template <class R>
+int32_t Ops<R>::yinv_vort(const scl_plan3d& pl, C* W, int64_t fstride) {
+  const Geom& g = pl.g;
+  return with_line<R>(g.ny, [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    using CC = ColCfg<R, N>;
+    SCL_CHECK((check_cols<R, N>(g)));
+    hipLaunchKernelGGL((k_yinv_vort<R, N>), dim3(col_grid<CC::CW, 1>(g), 4), dim3(CC::BS), 0, pl.stream, W, fstride,
+                       g, twp<R>(pl), pl.L / N);
+    return check_launch("yinv_vort");
+  });
+}
+
 // Column count of the fused z-pass kernels for a grid.  1024-point lines
 // stay at 8 columns: 16 would make 1024-thread workgroups, whose 128-VGPR
 // cap spills these kernels.
+#ifndef SCL_ZCOL16
+#define SCL_ZCOL16 1
+#endif
 template <class R>
 static bool zcol16(const Geom& g) {
-  return sizeof(R) == 4 && g.mx % 16 == 0 && g.ny % 16 == 0 && g.nz <= 512;
+  return SCL_ZCOL16 && sizeof(R) == 4 && g.mx % 16 == 0 && g.ny % 16 == 0 && g.nz <= 512;
+}
+
+// Points per thread of the 16-column fused z-pass kernels (0: the line
+// length's default).
+#ifndef SCL_Z16_E
+#define SCL_Z16_E 0
+#endif
+template <int N, int CW>
+constexpr int zpass_e() {
+  return (CW == 16 && SCL_Z16_E && N >= SCL_Z16_E) ? SCL_Z16_E : 0;
 }
 
 template <class R>
@@ -110,10 +136,11 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
     constexpr int N = decltype(NC)::value;
     auto go = [&](auto CWC) -> int32_t {
       constexpr int CW = decltype(CWC)::value;
-      using CC = ColCfg<R, N, CW>;
+      constexpr int EZ = zpass_e<N, CW>();
+      using CC = ColCfg<R, N, CW, EZ>;
       SCL_CHECK((check_cols<R, N, CW>(g)));
-      hipLaunchKernelGGL((k_zinv_vort<R, N, CW>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, sstride, w,
-                         wstride, g, twp<R>(pl), pl.L / N);
+      hipLaunchKernelGGL((k_zinv_vort<R, N, CW, EZ>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s,
+                         sstride, w, wstride, g, twp<R>(pl), pl.L / N);
       return check_launch("zinv_vort");
     };
     if constexpr (sizeof(R) == 4) {
@@ -125,23 +152,34 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
 
 template <class R>
 int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
-                         const double* coef, const RhsParams<R>& p) {
+                         const double* coef, const RhsParams<R>& p, bool fused) {
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
     auto go = [&](auto CWC) -> int32_t {
       constexpr int CW = decltype(CWC)::value;
-      using CC = ColCfg<R, N, CW>;
+      constexpr int EZ = zpass_e<N, CW>();
+      using CC = ColCfg<R, N, CW, EZ>;
       SCL_CHECK((check_cols<R, N, CW>(g)));
       const dim3 grd(col_grid<CW, 2>(g)), blk(CC::BS);
       const cpx<R>* tw = twp<R>(pl);
       const int tws = pl.L / N;
+      if (fused && mode >= 1) {
+        switch (mode) {
+          case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          default: set_error("bad RK stage %d", mode); return SCL_EINVAL;
+        }
+        return check_launch("zfwd_zinv");
+      }
       switch (mode) {
-        case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4, CW>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
         default: set_error("bad mode %d", mode); return SCL_EINVAL;
       }
       return check_launch("zfwd_rhs");

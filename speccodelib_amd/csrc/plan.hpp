@@ -45,6 +45,7 @@ struct scl_dns {
   double* coef = nullptr;    // forcing and Smagorinsky coefficients (8 doubles)
   double time = 0.0;
   int64_t steps = 0;
+  bool primed = false;       // W[0..4] holds the z-inverse of v (fused DNS stages)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipEvent_t> events;
