@@ -6,9 +6,10 @@ truncation), seeded k^4 exp(-2 k^2/16) initial field (iseed = 1, u0 = 1).
 A step is one Lawson RK4 step: 4 RHS evaluations (5 FFT/pointwise passes
 each) plus the stage updates, on fields resident in HBM.
 
-Prints ONE JSON line.  `value` is steps/s of the timed region (K steps
-bracketed by device synchronisation); per-kernel times come from HIP events
-recorded on the solver's stream around every launch of that same region.
+Prints ONE JSON line.  `value` is steps/s of the timed region: K steps of
+graph replay between device synchronisations.  The per-kernel times come
+from HIP events recorded on the solver's stream around every launch of the
+same K steps run again with direct launches.
 The roofline entry is for the kernel with the largest share of the step.
 cpu_baseline times the oracle (the FP32 numpy/scipy restatement of the same
 step) on the host's cores for one step at the same size.
@@ -33,19 +34,24 @@ def field_bytes(n):
     return 8 * (n // 2 + 1) * n * n
 
 
-# Algorithmic HBM bytes per launch, in units of F (one read or write of a
-# field counts F).  The forced-mode coefficient kernel and the kx=0-plane
-# Hermitian copy move O(N^2) bytes and count 0.
+# Algorithmic HBM bytes per launch of these kernels, in units of F (one read
+# or write of a field counts F).  The forced-mode coefficient kernel and the
+# kx=0-plane Hermitian copy move O(N^2) bytes and count 0.
 KIND_BYTES_F = {
     "force_coef": 0.0,
-    "zinv_vort": 9.0,    # read u (3), write u and w (6)
-    "y_inverse": 12.0,   # 6 fields read + written
+    "zinv_vort": 8.0,    # read u (3), write u, w_x, w_y (5)
+    "y_inverse": 11.0,   # read 5, write 6 (w_z formed in the pass)
     "x_pass": 9.0,       # read 6, write 3
     "y_forward": 6.0,    # 3 fields read + written
-    "zfwd_rk": 13.5,     # stages 1..4: 12, 15, 15, 12 (W, v, du in; du, S or v out)
+    "zfwd_rk": 12.75,    # stages 1..4: 12, 15, 15, 9 (W, v, du in; du, S or v out)
     "herm_fix": 0.0,
 }
-STEP_BYTES_F = 198.0  # 4 x (9 + 12 + 9 + 6) + (12 + 15 + 15 + 12)
+# The step model that the roofline fraction is priced against: one read +
+# write per axis pass (z-inverse 9, y-inverse 12, x-pass 9, y-forward 6;
+# z-forward + stage 12, 15, 15, 12).  These kernels move 187 F per step
+# (4 x (8 + 11 + 9 + 6) + 51).
+STEP_BYTES_F = 198.0
+IMPL_STEP_BYTES_F = 187.0
 
 
 def cpu_cores():
@@ -162,6 +168,26 @@ def bench_adjoint(scl, n=128, nsteps=200, ckpt=20, cs=0.15):
     }
 
 
+def copy_calibration(nbytes):
+    """One fill and one device-to-device copy of nbytes through the HIP runtime
+    that libscl.so loaded, so that a counter profile of this process holds two
+    kernels of known traffic (fill: nbytes written; copy: nbytes read and
+    written)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    if hip.hipMalloc(ctypes.byref(src), n) or hip.hipMalloc(ctypes.byref(dst), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, n)
+    hip.hipMemcpy(dst, src, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"[bench] copy calibration: fill + copy of {nbytes} bytes", file=sys.stderr, flush=True)
+
+
 def load_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
     if os.path.exists(path):
@@ -183,6 +209,10 @@ def main():
     ap.add_argument("--ens2d-member-steps", type=int, default=20)
     ap.add_argument("--no-ens2d", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="bytes of a device-to-device copy and a fill run after the timed region (counter "
+                         "calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding of this "
@@ -193,26 +223,29 @@ def main():
 
     n = a.n
     plan = scl.Plan3D(n, n, n, scl.SP)
-    dns = scl.DNS(plan, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, use_graph=True)
+    dns = scl.DNS(plan, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, use_graph=not a.eager)
     dns.initialize(1, 4.0, 1.0)
     dns.step(a.warmup)
     dns.sync()
-    # graph-replay timing (the production path)
+    # timed region: K steps of graph replay (the production path) between
+    # device synchronisations
     t0 = time.perf_counter()
     dns.step(a.steps)
     dns.sync()
-    graph_ms = (time.perf_counter() - t0) * 1e3 / a.steps
-    print(f"[bench] graph replay: {graph_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / a.steps
+    print(f"[bench] timed region (graph replay): {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
 
-    # timed region: K steps with HIP events around every launch
+    # the same K steps again with a HIP event around every launch on the
+    # solver's stream (direct launches: events inside a graph cannot be
+    # timed here), for the per-kernel times
     dns.sync()
-    t0 = time.perf_counter()
     ev_total_ms, kinds = dns.step_timed(a.steps)
     dns.sync()
-    wall_ms = (time.perf_counter() - t0) * 1e3
-    ms_per_step = wall_ms / a.steps
-    print(f"[bench] timed region: {ms_per_step:.3f} ms/step (events {ev_total_ms / a.steps:.3f})", file=sys.stderr, flush=True)
+    inst_ms = ev_total_ms / a.steps
+    print(f"[bench] instrumented steps: {inst_ms:.3f} ms/step", file=sys.stderr, flush=True)
     dg = dns.diagnostics()
+    if a.copy_calib:
+        copy_calibration(a.copy_calib)
 
     F = field_bytes(n)
     kern = {}
@@ -244,8 +277,10 @@ def main():
                    "force k^2<4 eps=0.1, 2/3 cut, IC k0=4 u0=1 iseed=1", "n": n, "parallelism": "single GPU"},
         "hbm_roofline_frac_step": step_bytes / (ms_per_step * 1e-3) / HBM_PEAK,
         "step_alg_bytes": step_bytes,
+        "impl_step_bytes": IMPL_STEP_BYTES_F * F,
+        "impl_bandwidth_frac_step": IMPL_STEP_BYTES_F * F / (ms_per_step * 1e-3) / HBM_PEAK,
         "ns_per_point_step": ms_per_step * 1e6 / n**3,
-        "graph_ms_per_step": graph_ms,
+        "instrumented_ms_per_step": inst_ms,
         "adjoint_forward_ratio": None,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": load_traffic(dom, n)},

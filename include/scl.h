@@ -206,8 +206,9 @@ int32_t scl_dns_helicity_spectrum(scl_dns *d, double *hk, int32_t nbins);
 int32_t scl_dns_diagnostics(scl_dns *d, scl_diag *out);
 
 /* Per-kernel timing of nsteps steps with HIP events on the plan's stream
- * (non-graph launches).  kind_ms[k] / kind_count[k] for k < SCL_NKIND,
- * total_ms = event time of the whole run. */
+ * around every launch (direct launches, also when use_graph is set: event
+ * records captured into a graph cannot be timed).  kind_ms[k] /
+ * kind_count[k] for k < SCL_NKIND, total_ms = event time of the whole run. */
 enum {
   SCL_K_FORCE = 0, SCL_K_ZINV = 1, SCL_K_YINV = 2, SCL_K_XPASS = 3, SCL_K_YFWD = 4, SCL_K_ZFWD = 5,
   SCL_K_HERM = 6, SCL_NKIND = 7

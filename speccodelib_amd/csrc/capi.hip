@@ -529,7 +529,7 @@ struct Timer {
 // (the fused five-pass DNS RHS; nc = 3).
 template <class R>
 static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsParams<R>& rp, double eps, double eta,
-                        double fkmax, const cpx<R>* st, Timer* tm, StageChain* chain = nullptr) {
+                        double fkmax, const cpx<R>* st, Timer* tm) {
   const int64_t fs = d.fs;
   cpx<R>* V = d.V<R>();
   cpx<R>* DU = d.DU<R>();
@@ -537,17 +537,12 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   cpx<R>* W = d.W<R>();
   Timer none;
   if (!tm) tm = &none;
-  // the fused z-forward writes S only on the forced modes: not when the
-  // caller collects the stage states (s_out)
-  const bool fused = chain && chain->fused && mode >= 1 && !d.s_out;
   if (rp.forced) {
     tm->mark(SCL_K_FORCE);
     SCL_CHECK(Ops<R>::force_coef(*p, st, fs, fkmax, eps, eta, rp.forced, d.coef));
   }
-  if (!(chain && chain->primed)) {
-    tm->mark(SCL_K_ZINV);
-    SCL_CHECK(Ops<R>::zinv_vort(*p, st, fs, W, fs));
-  }
+  tm->mark(SCL_K_ZINV);
+  SCL_CHECK(Ops<R>::zinv_vort(*p, st, fs, W, fs));
   tm->mark(SCL_K_YINV);
   SCL_CHECK(Ops<R>::yinv_vort(*p, W, fs));
   tm->mark(SCL_K_XPASS);
@@ -555,9 +550,14 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   tm->mark(SCL_K_YFWD);
   SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
   tm->mark(SCL_K_ZFWD);
-  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp, fused));
-  if (chain) chain->primed = fused;
-  if (fused && mode < 4) {
+  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp));
+  // The kx = 0 Hermitian copy (symmetrize.f90:51-61) goes on N (mode 0),
+  // on the collected stage states (s_out: the adjoint's tape), and on v once
+  // per step.  Skipping it on du and the stage state of stages 1..3 changes
+  // them only at rounding level: the kx = 0 plane of N is Hermitian up to
+  // rounding (it is the transform of real products), and c2r ignores the
+  // anti-Hermitian part of the kx = 0 lines.
+  if (mode >= 1 && mode < 4 && !d.s_out) {
     tm->mark(-1);
     return SCL_OK;
   }
@@ -582,12 +582,11 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
 }
 
 template <class R>
-static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm,
-                            StageChain* chain = nullptr) {
+static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm) {
   const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
   for (int stage = 1; stage <= 4; ++stage) {
     const cpx<R>* st = stage == 1 ? d.V<R>() : d.S<R>();
-    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm, chain));
+    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm));
   }
   return SCL_OK;
 }
@@ -613,6 +612,11 @@ static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const s
   tm->mark(SCL_K_ZFWD);
   SCL_CHECK(ExtOps<R>::rhs(*p, mode, L, d.W<R>(), fs, d.V<R>(), d.DU<R>(), d.S<R>(), st, d.coef, ep,
                            (R)(delta * delta)));
+  // the kx = 0 Hermitian copy: as in rk_stage
+  if (mode >= 1 && mode < 4 && !d.s_out) {
+    tm->mark(-1);
+    return SCL_OK;
+  }
   tm->mark(SCL_K_HERM);
   FieldPtrs<R> fp{};
   const int nc = d.nc;
@@ -635,19 +639,15 @@ static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const s
 }
 
 template <class R>
-int32_t scl::solver_stage(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st,
-                          StageChain* chain) {
-  if (prm.sgs || prm.scalar) {
-    if (chain) chain->primed = false;
-    return rk_stage_ext<R>(p, d, mode, prm, st, nullptr);
-  }
+int32_t scl::solver_stage(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st) {
+  if (prm.sgs || prm.scalar) return rk_stage_ext<R>(p, d, mode, prm, st, nullptr);
   const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
-  return rk_stage<R>(p, d, mode, rp, prm.eps, prm.eta, prm.fkmax, st, nullptr, chain);
+  return rk_stage<R>(p, d, mode, rp, prm.eps, prm.eta, prm.fkmax, st, nullptr);
 }
 template int32_t scl::solver_stage<float>(scl_plan3d*, const DevFields&, int, const scl_dns_params&,
-                                          const cpx<float>*, StageChain*);
+                                          const cpx<float>*);
 template int32_t scl::solver_stage<double>(scl_plan3d*, const DevFields&, int, const scl_dns_params&,
-                                           const cpx<double>*, StageChain*);
+                                           const cpx<double>*);
 
 extern "C" int32_t scl_ratesdudt(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* nx_, void* ny_,
                                  void* nz_, int32_t forced, double fkmax, double eps, double eta) {
@@ -783,7 +783,6 @@ extern "C" int32_t scl_dns_destroy(scl_dns* d) {
 
 static int32_t dns_io(scl_dns* d, int first, int n, const void* const* in, void* const* out) {
   scl_plan3d* p = d->plan;
-  if (in) d->primed = false;  // the state changes: W no longer holds its z-inverse
   return by_prec(p->prec, [&](auto zero) -> int32_t {
     using R = decltype(zero);
     void* s;
@@ -840,12 +839,7 @@ template <class R>
 static int32_t dns_step_once(scl_dns* d, Timer* tm) {
   scl_plan3d* p = d->plan;
   DevFields f = dev_fields(d);
-  if (!d->ext) {
-    StageChain chain{true, d->primed};
-    const int32_t st = rk4_step_dev<R>(p, f, d->p, tm, &chain);
-    d->primed = st == SCL_OK && chain.primed;
-    return st;
-  }
+  if (!d->ext) return rk4_step_dev<R>(p, f, d->p, tm);
   const scl_dns_params& prm = d->p;
   if (prm.sgs == 2 && d->steps % (prm.iupdate > 0 ? prm.iupdate : 1) == 0) {
     const double delta = prm.delta > 0 ? prm.delta : 2.0 * M_PI / p->g.nx;
@@ -864,12 +858,6 @@ template <class R>
 static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
   scl_plan3d* p = d->plan;
   if (d->p.use_graph && !tm) {
-    // the captured step starts from a primed W (z-inverse of v in W[0..4])
-    if (!d->ext && !d->primed) {
-      DevFields f = dev_fields(d);
-      SCL_CHECK(Ops<R>::zinv_vort(*p, f.V<R>(), f.fs, f.W<R>(), f.fs));
-      d->primed = true;
-    }
     if (!d->exec) {
       SCL_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
       const int64_t keep = d->steps;

@@ -104,7 +104,6 @@ template <class R>
 int32_t rhs_split(scl_dns* d, const cpx<R>* st, cpx<R>* out, bool linear) {
   scl_plan3d* p = d->plan;
   DevFields f{d->fields, p->fstride(), d->nc, d->coef};
-  d->primed = false;  // the work fields are overwritten
   SCL_CHECK(solver_stage<R>(p, f, 0, d->p, st));
   const R nu = linear ? (R)d->p.nu : (R)0;
   const R nu_phi = linear ? (R)(d->p.nu / (d->p.schmidt > 0 ? d->p.schmidt : 1.0)) : (R)0;
@@ -163,7 +162,6 @@ extern "C" int32_t scl_dns_state(scl_dns* d, void** state, int64_t* n) {
   if (!d || !state || !n) return set_error("null argument"), SCL_EINVAL;
   *state = d->fields;
   *n = (int64_t)d->nc * d->plan->fstride();
-  d->primed = false;  // the caller may change the state through the pointer
   return SCL_OK;
 }
 

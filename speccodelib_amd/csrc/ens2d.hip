@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "ic.hpp"
+#include "plan.hpp"
 #include "vort2d.hpp"
 
 namespace scl {
@@ -32,7 +33,7 @@ struct Group {
   int e = 8, cw = 8, row_grid = 0;
   hipStream_t stream = nullptr;  // branch stream (batched mode)
   hipEvent_t done = nullptr;
-  int64_t fs() const { return g.ntot; }
+  int64_t fs() const { return g.ntot + SCL_FIELD_PAD; }  // padded as the 3D fields
   template <class R>
   cpx<R>* field(int i) const {
     return reinterpret_cast<cpx<R>*>(base) + i * fs();

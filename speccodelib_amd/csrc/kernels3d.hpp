@@ -270,40 +270,27 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
 // One IF-RK4 stage of one component in the division-free form (see
 // k_zfwd_rhs): a = dt N, em1 = expm1(x), x = -rate dt/2, so E y = y + em1 y
 // and E^2 y = y + expm1(2x) y, each exact to the rounding of the result.
-// Returns the new stage state (MODE 1..3) or the new v (MODE 4); the stage
-// state is stored only where store_s.
 template <int MODE, class R>
-__device__ inline cpx<R> stage_keep(cpx<R> a, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
-                                    bool store_s) {
-  cpx<R> out;
+__device__ inline void stage_update(cpx<R> a, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
   if constexpr (MODE == 1) {
     const cpx<R> v = V[o];
     const cpx<R> u = v + ((R)1 / (R)6) * a, q = v + (R)0.5 * a;
     DU[o] = u + expm1_r((R)2 * x) * u;
-    out = q + em1 * q;
+    S[o] = q + em1 * q;
   } else if constexpr (MODE == 2 || MODE == 3) {
     const cpx<R> v = V[o];
     const cpx<R> q = ((R)1 / (R)3) * a;
     DU[o] = DU[o] + (q + em1 * q);
     const cpx<R> ev = v + em1 * v;
     if constexpr (MODE == 2) {
-      out = ev + (R)0.5 * a;
+      S[o] = ev + (R)0.5 * a;
     } else {
       const cpx<R> u = ev + a;
-      out = u + em1 * u;
+      S[o] = u + em1 * u;
     }
   } else {
-    out = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
-    V[o] = out;
-    return out;
+    V[o] = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
   }
-  if (store_s) S[o] = out;
-  return out;
-}
-
-template <int MODE, class R>
-__device__ inline void stage_update(cpx<R> a, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
-  stage_keep<MODE, R>(a, o, V, DU, S, em1, x, cut, true);
 }
 
 // Parameters of the fused ZFWD epilogue.
@@ -331,11 +318,7 @@ struct RhsParams {
 //   MODE 3: du += E a/3;         S = E (E v + a)
 //   MODE 4: v = du + a/6, 0 where k2 >= kc2
 // so the last stage reads no v, and no stage divides by E.
-// FUSED (MODE 1..4): the new stage state (or v) stays in registers and its
-// z-inverse goes to W[0..4] for the next RHS (zinv_store), so that RHS
-// needs no z-inverse pass of its own; S is then stored only where the
-// forcing reads it (k2 < fk2).
-template <class R, int N, int MODE, int CWZ, int EZ = 0, bool FUSED = false>
+template <class R, int N, int MODE, int CWZ, int EZ = 0>
 __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V,
                                                                     cpx<R>* DU, cpx<R>* S, const cpx<R>* st,
                                                                     const double* __restrict__ coef, RhsParams<R> p,
@@ -421,21 +404,10 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R x = -p.nu * k2 * p.dt / (R)2;
       const R em1 = expm1_r(x);
       const bool cut = k2 >= p.kc2;
-      const bool store_s = !FUSED || (p.forced && k2 < p.fk2);
-      nx_[m] = stage_keep<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut, store_s);
-      ny_[m] = stage_keep<MODE, R>(p.dt * b, o + fstride, V, DU, S, em1, x, cut, store_s);
-      nz_[m] = stage_keep<MODE, R>(p.dt * d, o + 2 * fstride, V, DU, S, em1, x, cut, store_s);
+      stage_update<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut);
+      stage_update<MODE, R>(p.dt * b, o + fstride, V, DU, S, em1, x, cut);
+      stage_update<MODE, R>(p.dt * d, o + 2 * fstride, V, DU, S, em1, x, cut);
     }
-  }
-  if constexpr (FUSED && MODE > 0) {
-    int64_t off[E];
-    R kzs[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      off[m] = t.base + (int64_t)(rt + T * m) * t.rs;
-      kzs[m] = (R)kwave(rt + T * m, g.nz);
-    }
-    zinv_store<R, N, E, CW>(nx_, ny_, nz_, lds, twl, rt, c, kx, ky, off, kzs, W, fstride);
   }
 }
 

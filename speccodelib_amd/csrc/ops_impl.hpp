@@ -152,7 +152,7 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
 
 template <class R>
 int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
-                         const double* coef, const RhsParams<R>& p, bool fused) {
+                         const double* coef, const RhsParams<R>& p) {
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
@@ -164,16 +164,6 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
       const dim3 grd(col_grid<CW, 2>(g)), blk(CC::BS);
       const cpx<R>* tw = twp<R>(pl);
       const int tws = pl.L / N;
-      if (fused && mode >= 1) {
-        switch (mode) {
-          case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-          case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-          case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-          case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-          default: set_error("bad RK stage %d", mode); return SCL_EINVAL;
-        }
-        return check_launch("zfwd_zinv");
-      }
       switch (mode) {
         case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
         case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;

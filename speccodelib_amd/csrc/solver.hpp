@@ -31,23 +31,10 @@ struct DevFields {
   cpx<R>* W() const { return f<R>(3 * nc); }
 };
 
-// Chaining of the DNS stages: with fused set, each stage's z-forward also
-// leaves the z-inverse of the next stage state in W[0..4] (primed), so the
-// next stage skips its z-inverse pass; the stage state S is then stored
-// only on the forced modes, and the kx = 0 Hermitian copy is applied to v
-// once per step.  primed must be false whenever W[0..4] does not hold the
-// z-inverse of st (after any change to v or W outside the stages).
-struct StageChain {
-  bool fused = false;
-  bool primed = false;
-};
-
 // IF-RK4 stage `mode` (1..4; 0: N(st) into W) of the solver configuration
 // prm on the fields d, reading the stage state st: the fused DNS RHS, or
-// the multi-pass RHS when prm asks for a Smagorinsky term or a scalar
-// (chain ignored there).  chain == nullptr: every stage stands alone.
+// the multi-pass RHS when prm asks for a Smagorinsky term or a scalar.
 template <class R>
-int32_t solver_stage(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st,
-                     StageChain* chain = nullptr);
+int32_t solver_stage(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st);
 
 }  // namespace scl
