@@ -188,6 +188,44 @@ def copy_calibration(nbytes):
     print(f"[bench] copy calibration: fill + copy of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
+# 1024^3 FP32 DNS with helical forcing (force_heli.f90): nu = 2.5e-4,
+# dt = 2.5e-4, forcing on k^2 < 4 with eps = 0.1 and eta = 2 k_f eps
+# (relative helicity 1 at k_f = 2), seeded k^4 exp(-2 (k/4)^2) field.
+def bench_heli1024(scl, steps, n=1024):
+    plan = scl.Plan3D(n, n, n, scl.SP)
+    d = scl.DNS(plan, nu=2.5e-4, dt=2.5e-4, forced=2, fkmax=2.0, eps=0.1, eta=0.4, use_graph=True)
+    t0 = time.perf_counter()
+    d.initialize(1, 4.0, 1.0)
+    d.sync()
+    ic_s = time.perf_counter() - t0
+    d.step(1)
+    d.sync()
+    t0 = time.perf_counter()
+    d.step(steps)
+    d.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    _, kinds = d.step_timed(2)
+    dg = d.diagnostics()
+    ek, hk = d.spectrum(), d.helicity_spectrum()
+    d.close()
+    plan.close()
+    F = field_bytes(n)
+    print(f"[bench] helical {n}^3: {ms:.1f} ms/step", file=sys.stderr, flush=True)
+    return {
+        "workload": f"helical forced DNS {n}^3 FP32, nu=2.5e-4, dt=2.5e-4, force_heli k^2<4 eps=0.1 eta=0.4",
+        "steps": steps,
+        "ms_per_step": ms,
+        "steps_per_s": 1e3 / ms,
+        "hbm_roofline_frac_step": STEP_BYTES_F * F / (ms * 1e-3) / HBM_PEAK,
+        "kernel_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in kinds.items()},
+        "initial_field_s": ic_s,
+        "energy": dg["energy"],
+        "helicity": dg["helicity"],
+        "Ek_1_8": [float(x) for x in ek[:8]],
+        "Hk_1_8": [float(x) for x in hk[:8]],
+    }
+
+
 def load_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
     if os.path.exists(path):
@@ -196,31 +234,8 @@ def load_traffic(kernel, n):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph-check", action="store_true")
-    ap.add_argument("--ens2d-steps", type=int, default=1000)
-    ap.add_argument("--ens2d-member-steps", type=int, default=20)
-    ap.add_argument("--no-ens2d", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
-    ap.add_argument("--copy-calib", type=int, default=0,
-                    help="bytes of a device-to-device copy and a fill run after the timed region (counter "
-                         "calibration)")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding of this "
-              "path is a design, not a build", file=sys.stderr)
-        sys.exit(2)
-
-    import speccodelib_amd as scl
-
+def bench_main(scl, a):
+    """The 512^3 forced DNS: the headline line (everything but the extra sections)."""
     n = a.n
     plan = scl.Plan3D(n, n, n, scl.SP)
     dns = scl.DNS(plan, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, use_graph=not a.eager)
@@ -292,13 +307,49 @@ def main():
         sys.exit(3)
     del dns
     plan.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph-check", action="store_true")
+    ap.add_argument("--ens2d-steps", type=int, default=1000)
+    ap.add_argument("--ens2d-member-steps", type=int, default=20)
+    ap.add_argument("--no-ens2d", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--heli1024", type=int, default=0, help="also time this many steps of the 1024^3 helical DNS")
+    ap.add_argument("--no-main", action="store_true", help="skip the 512^3 DNS (profiling runs of the others)")
+    ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="bytes of a device-to-device copy and a fill run after the timed region (counter "
+                         "calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding of this "
+              "path is a design, not a build", file=sys.stderr)
+        sys.exit(2)
+
+    import speccodelib_amd as scl
+
+    if a.no_main:
+        out = {"metric": METRIC, "value": None, "unit": "steps/s", "note": "512^3 DNS skipped (--no-main)"}
+    else:
+        out = bench_main(scl, a)
     out["ens2d"] = None if a.no_ens2d else bench_ens2d(scl, a.ens2d_steps, a.ens2d_member_steps)
     if not a.no_adjoint:
         out["adjoint"] = bench_adjoint(scl)
         out["adjoint_forward_ratio"] = out["adjoint"]["ratio"]
+    if a.heli1024:
+        out["heli1024"] = bench_heli1024(scl, a.heli1024)
     if not a.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(n, a.cpu_steps, scl)
+        out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, scl)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -118,8 +118,9 @@ struct scl_ens2d {
   std::vector<std::pair<int, void*>> tw;    // twiddle table per size
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};      // one step; a chunk of steps
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int chunk = 10;  // steps per chunk graph (batched mode; 1 per member)
   double* diag = nullptr;
   int ncu = 256;
   int64_t steps = 0;
@@ -210,13 +211,18 @@ int32_t group_step(const scl_ens2d& e, const Group& gr, hipStream_t st) {
 
 // One step of every group.  Batched: the groups run on parallel branches
 // (fork/join through events, which graph capture turns into graph edges).
-int32_t launch_step(scl_ens2d& e) {
+// nsteps steps of every group.  Batched: the groups are independent, so each
+// runs its nsteps steps as one chain on its own branch (fork/join through
+// events, which graph capture turns into graph edges) and the branches
+// join once, at the end.
+int32_t launch_steps(scl_ens2d& e, int nsteps) {
   const bool fork = !e.p.per_member && e.groups.size() > 1;
   if (fork) SCL_HIP(hipEventRecord(e.fork, e.stream));
   for (auto& gr : e.groups) {
     hipStream_t st = fork ? gr.stream : e.stream;
     if (fork) SCL_HIP(hipStreamWaitEvent(st, e.fork, 0));
-    SCL_CHECK(e.prec == SCL_SP ? group_step<float>(e, gr, st) : group_step<double>(e, gr, st));
+    for (int s = 0; s < nsteps; ++s)
+      SCL_CHECK(e.prec == SCL_SP ? group_step<float>(e, gr, st) : group_step<double>(e, gr, st));
     if (fork) {
       SCL_HIP(hipEventRecord(gr.done, st));
       SCL_HIP(hipStreamWaitEvent(e.stream, gr.done, 0));
@@ -226,17 +232,19 @@ int32_t launch_step(scl_ens2d& e) {
 }
 
 void drop_graph(scl_ens2d& e) {
-  if (e.exec) hipGraphExecDestroy(e.exec);
-  if (e.graph) hipGraphDestroy(e.graph);
-  e.exec = nullptr;
-  e.graph = nullptr;
+  for (int i = 0; i < 2; ++i) {
+    if (e.exec[i]) hipGraphExecDestroy(e.exec[i]);
+    if (e.graph[i]) hipGraphDestroy(e.graph[i]);
+    e.exec[i] = nullptr;
+    e.graph[i] = nullptr;
+  }
 }
 
-int32_t run_steps(scl_ens2d& e, int nsteps) {
-  if (nsteps < 0) return set_error("nsteps < 0"), SCL_EINVAL;
-  if (e.p.use_graph && !e.exec && nsteps > 0) {
+// Graph i of e (0: one step, 1: a chunk of e.chunk steps), built on first use.
+int32_t graph_exec(scl_ens2d& e, int i, hipGraphExec_t* out) {
+  if (!e.exec[i]) {
     SCL_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    const int32_t st = launch_step(e);
+    const int32_t st = launch_steps(e, i ? e.chunk : 1);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e.stream, &g);
     if (st != SCL_OK) {
@@ -244,15 +252,32 @@ int32_t run_steps(scl_ens2d& e, int nsteps) {
       return st;
     }
     SCL_HIP(ce);
-    e.graph = g;
-    SCL_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+    e.graph[i] = g;
+    SCL_HIP(hipGraphInstantiate(&e.exec[i], e.graph[i], nullptr, nullptr, 0));
   }
-  for (int s = 0; s < nsteps; ++s) {
-    if (e.exec) SCL_HIP(hipGraphLaunch(e.exec, e.stream));
-    else SCL_CHECK(launch_step(e));
-    e.steps += 1;
-    e.time += e.p.dt;
+  *out = e.exec[i];
+  return SCL_OK;
+}
+
+int32_t run_steps(scl_ens2d& e, int nsteps) {
+  if (nsteps < 0) return set_error("nsteps < 0"), SCL_EINVAL;
+  int done = 0;
+  if (e.p.use_graph) {
+    hipGraphExec_t x;
+    if (e.chunk > 1 && nsteps >= e.chunk) {
+      SCL_CHECK(graph_exec(e, 1, &x));
+      for (; done + e.chunk <= nsteps; done += e.chunk) SCL_HIP(hipGraphLaunch(x, e.stream));
+    }
+    if (done < nsteps) {
+      SCL_CHECK(graph_exec(e, 0, &x));
+      for (; done < nsteps; ++done) SCL_HIP(hipGraphLaunch(x, e.stream));
+    }
+  } else {
+    SCL_CHECK(launch_steps(e, nsteps));
+    done = nsteps;
   }
+  e.steps += done;
+  e.time += done * e.p.dt;
   return SCL_OK;
 }
 
@@ -389,6 +414,9 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
   auto* e = new scl_ens2d();
   e->prec = prec;
   e->p = *params;
+  // per-member mode keeps one step per graph: a chunk would hold
+  // chunk x members x 12 nodes
+  e->chunk = params->per_member ? 1 : 10;
   auto fail = [&](int32_t st) {
     scl_ens2d_destroy(e);
     return st;

@@ -270,8 +270,8 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
 // One IF-RK4 stage of one component in the division-free form (see
 // k_zfwd_rhs): a = dt N, em1 = expm1(x), x = -rate dt/2, so E y = y + em1 y
 // and E^2 y = y + expm1(2x) y, each exact to the rounding of the result.
-template <int MODE, class R>
-__device__ inline void stage_update(cpx<R> a, int64_t o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
+template <int MODE, class R, class I>
+__device__ inline void stage_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
   if constexpr (MODE == 1) {
     const cpx<R> v = V[o];
     const cpx<R> u = v + ((R)1 / (R)6) * a, q = v + (R)0.5 * a;
@@ -330,13 +330,19 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
   for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  // Entry offsets within a field fit in 32 bits (< 2^30 at 1024^3); the
+  // three components go through their own base pointers, so a thread keeps
+  // one 32-bit offset per slot instead of 64-bit ones (which spilled).
+  const uint32_t ob = (uint32_t)t.base, rs = (uint32_t)t.rs;
+  cpx<R>* const W1 = W + fstride;
+  cpx<R>* const W2 = W + 2 * fstride;
   cpx<R> nx_[E], ny_[E], nz_[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int64_t o = t.base + (int64_t)(rt + T * m) * t.rs;
+    const uint32_t o = ob + (uint32_t)(rt + T * m) * rs;
     nx_[m] = W[o];
-    ny_[m] = W[o + fstride];
-    nz_[m] = W[o + 2 * fstride];
+    ny_[m] = W1[o];
+    nz_[m] = W2[o];
   }
   using L = Line<R, N, E, -1>;
   L::run(nx_, lds, rt, twl, 1, ColIdx<CW>{c});
@@ -359,11 +365,11 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     const R kz = (R)ikz;
     const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
     const R k2 = ik2 == 0 ? tiny : (R)ik2;
-    const int64_t o = t.base + (int64_t)z * t.rs;
+    const uint32_t o = ob + (uint32_t)z * rs;
     cpx<R> a = nx_[m], b = ny_[m], d = nz_[m];
     // forcing (force.f90:25-29 / force_heli.f90:44-52)
     if (p.forced && k2 < p.fk2) {
-      const cpx<R> sx = st[o], sy = st[o + fstride], sz = st[o + 2 * fstride];
+      const cpx<R> sx = st[o], sy = st[fstride + o], sz = st[2 * fstride + o];
       a = a + fa * sx;
       b = b + fa * sy;
       d = d + fa * sz;
@@ -394,8 +400,8 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     }
     if constexpr (MODE == 0) {
       W[o] = a;
-      W[o + fstride] = b;
-      W[o + 2 * fstride] = d;
+      W1[o] = b;
+      W2[o] = d;
     } else {
       // E = exp(x), x = -nu k2 dt/2.  E is within 1e-3 of 1 for most modes,
       // where FP32 spacing (6e-8) makes a rounded E or E*E a biased factor
@@ -405,8 +411,8 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R em1 = expm1_r(x);
       const bool cut = k2 >= p.kc2;
       stage_update<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut);
-      stage_update<MODE, R>(p.dt * b, o + fstride, V, DU, S, em1, x, cut);
-      stage_update<MODE, R>(p.dt * d, o + 2 * fstride, V, DU, S, em1, x, cut);
+      stage_update<MODE, R>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut);
+      stage_update<MODE, R>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1, x, cut);
     }
   }
 }

@@ -88,30 +88,37 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yinv2d(const cpx<R
   const R kx = (R)t.kx;
   const R tiny = std::numeric_limits<R>::min();
   using L = Line<R, N, E, +1>;
-  cpx<R> u[E], v[E], o[E];
+  // one field is formed, transformed and stored at a time, so at most two
+  // columns of E points are live per thread
+  cpx<R> o[E], a[E];
+  R ik2r[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int y = rt + T * m;
     o[m] = t.ok ? s[t.base + (int64_t)y * t.rs] : mk<R>(0, 0);
     const int iky = kwave(y, N);
     const int ik2 = t.kx * t.kx + iky * iky;
-    const R k2 = ik2 == 0 ? tiny : (R)ik2;
-    const R ky = (R)iky;
-    // u = i ky w/k^2, v = -i kx w/k^2 (spectral_dns_2d.f90:161-162)
-    u[m] = mk<R>(-(ky * o[m].y) / k2, (ky * o[m].x) / k2);
-    v[m] = mk<R>((kx * o[m].y) / k2, -(kx * o[m].x) / k2);
+    ik2r[m] = ik2 == 0 ? tiny : (R)ik2;
   }
-  L::run(u, lds, rt, twl, 1, ColIdx<CW>{c});
-  L::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
-  L::run(o, lds, rt, twl, 1, ColIdx<CW>{c});
-  if (!t.ok) return;
+  auto store = [&](cpx<R>(&v)[E], int f) {
+    if (!t.ok) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[t.base + (int64_t)L::out_pos(m, rt) * t.rs + f * fstride] = v[m];
+  };
+  // u = i ky w/k^2, v = -i kx w/k^2 (spectral_dns_2d.f90:161-162)
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int64_t off = t.base + (int64_t)L::out_pos(m, rt) * t.rs;
-    w[off] = u[m];
-    w[off + fstride] = v[m];
-    w[off + 2 * fstride] = o[m];
+    const R ky = (R)kwave(rt + T * m, N);
+    a[m] = mk<R>(-(ky * o[m].y) / ik2r[m], (ky * o[m].x) / ik2r[m]);
   }
+  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(a, 0);
+#pragma unroll
+  for (int m = 0; m < E; ++m) a[m] = mk<R>((kx * o[m].y) / ik2r[m], -(kx * o[m].x) / ik2r[m]);
+  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(a, 1);
+  L::run(o, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(o, 2);
 }
 
 // XPASS: per row, c2r of u, v, w; u w and v w times scale; r2c of both
@@ -208,7 +215,23 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
   const R tiny = std::numeric_limits<R>::min();
   const int ikx = t.kx;
   const R kx = (R)ikx;
-  cpx<R> o1[E], o2[E];
+  // Outputs go straight to memory.  In the tile holding the kx = 0 column,
+  // lane c = 0 also stages its outputs in LDS; after a barrier it rewrites
+  // its entries y > N/2 as c(0, -ky) = conj c(0, ky) (symmetrize.f90:25-27).
+  // That equals symmetrizing N first: every output is a real-linear function
+  // of N and of inputs that already have this symmetry.
+  if (kx0_tile) __syncthreads();  // the transforms' LDS reads are done
+  auto put = [&](uint32_t y, cpx<R> v1, cpx<R> v2) {
+    const int64_t o = t.base + (int64_t)y * t.rs;
+    if (mode == 0) {
+      W[o] = v1;
+    } else if (mode == 4) {
+      WS[o] = v1;
+    } else {
+      ACC[o] = v1;
+      S[o] = v2;
+    }
+  };
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int y = L::out_pos(m, rt);
@@ -229,69 +252,44 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
     }
     // symmetrize2d (symmetrize.f90:17-22) and the 2/3 cut
     if (y == N / 2 || t.kx == g.mx || ik2 == 0 || k2 >= p.kc2) r = mk<R>(0, 0);
-    if (mode == 0 || !t.ok) {
-      o1[m] = r;
-      o2[m] = mk<R>(0, 0);
-      continue;
-    }
-    const R x = -p.rate(k2) * p.dt / (R)2;
-    const R em1 = expm1_r(x);
-    const cpx<R> wv = WS[o];
-    const cpx<R> a = p.dt * r;
-    if (mode == 1) {
-      const cpx<R> u = wv + ((R)1 / (R)6) * a, q = wv + (R)0.5 * a;
-      o1[m] = u + expm1_r((R)2 * x) * u;
-      o2[m] = q + em1 * q;
-    } else if (mode == 2 || mode == 3) {
-      const cpx<R> q = ((R)1 / (R)3) * a;
-      o1[m] = ACC[o] + (q + em1 * q);
-      const cpx<R> ew = wv + em1 * wv;
-      if (mode == 2) {
-        o2[m] = ew + (R)0.5 * a;
+    cpx<R> o1 = r, o2 = mk<R>(0, 0);
+    if (mode != 0 && t.ok) {
+      const R x = -p.rate(k2) * p.dt / (R)2;
+      const R em1 = expm1_r(x);
+      const cpx<R> wv = WS[o];
+      const cpx<R> a = p.dt * r;
+      if (mode == 1) {
+        const cpx<R> u = wv + ((R)1 / (R)6) * a, q = wv + (R)0.5 * a;
+        o1 = u + expm1_r((R)2 * x) * u;
+        o2 = q + em1 * q;
+      } else if (mode == 2 || mode == 3) {
+        const cpx<R> q = ((R)1 / (R)3) * a;
+        o1 = ACC[o] + (q + em1 * q);
+        const cpx<R> ew = wv + em1 * wv;
+        if (mode == 2) {
+          o2 = ew + (R)0.5 * a;
+        } else {
+          const cpx<R> u = ew + a;
+          o2 = u + em1 * u;
+        }
       } else {
-        const cpx<R> u = ew + a;
-        o2[m] = u + em1 * u;
+        o1 = k2 >= p.kc2 ? mk<R>(0, 0) : ACC[o] + ((R)1 / (R)6) * a;
       }
-    } else {
-      o1[m] = k2 >= p.kc2 ? mk<R>(0, 0) : ACC[o] + ((R)1 / (R)6) * a;
+    }
+    if (t.ok) put(y, o1, o2);
+    if (kx0_tile && c == 0) {
+      lds[y] = o1;
+      lds[N + y] = o2;
     }
   }
-  // kx = 0 column: c(0, -ky) = conj c(0, ky) (symmetrize.f90:25-27), applied
-  // to the outputs, which equals symmetrizing N first: every output is a
-  // real-linear function of N and of inputs that already have this symmetry.
   if (kx0_tile) {
     __syncthreads();
     if (c == 0) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int y = L::out_pos(m, rt);
-        lds[y] = o1[m];
-        lds[N + y] = o2[m];
+        if (y > N / 2) put(y, conj(lds[N - y]), conj(lds[2 * N - y]));
       }
-    }
-    __syncthreads();
-    if (c == 0) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int y = L::out_pos(m, rt);
-        if (y > N / 2) {
-          o1[m] = conj(lds[N - y]);
-          o2[m] = conj(lds[2 * N - y]);
-        }
-      }
-    }
-  }
-  if (!t.ok) return;
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int64_t o = t.base + (int64_t)L::out_pos(m, rt) * t.rs;
-    if (mode == 0) {
-      W[o] = o1[m];
-    } else if (mode == 4) {
-      WS[o] = o1[m];
-    } else {
-      ACC[o] = o1[m];
-      S[o] = o2[m];
     }
   }
 }
