@@ -45,13 +45,28 @@ KIND_BYTES_F = {
     "y_forward": 6.0,    # 3 fields read + written
     "zfwd_rk": 12.75,    # stages 1..4: 12, 15, 15, 9 (W, v, du in; du, S or v out)
     "herm_fix": 0.0,
+    # z-forward + stage + z-inverse of the next stage state (stage chain):
+    # stages 1..3: 14, 17, 17 (W, v, du in; du and W[0..4] out)
+    "zfwd_zinv": 16.0,
 }
 # The step model that the roofline fraction is priced against: one read +
 # write per axis pass (z-inverse 9, y-inverse 12, x-pass 9, y-forward 6;
-# z-forward + stage 12, 15, 15, 12).  These kernels move 187 F per step
-# (4 x (8 + 11 + 9 + 6) + 51).
+# z-forward + stage 12, 15, 15, 12).
 STEP_BYTES_F = 198.0
-IMPL_STEP_BYTES_F = 187.0
+# What these kernels move per step: one z-inverse (8), 4 x (11 + 9 + 6) for
+# the y-inverse, x-pass and y-forward, stages 1..3 fused with the next
+# z-inverse (14 + 17 + 17) and the stage-4 z-forward (9): 169 F.
+IMPL_STEP_BYTES_F = 169.0
+
+
+def kind_bytes_f(kinds):
+    """Per-launch bytes of each kind in one step_timed run: with stage
+    chaining the unfused z-forward is stage 4 only."""
+    out = dict(KIND_BYTES_F)
+    if kinds.get("zfwd_zinv", (0.0, 0))[1]:
+        out["zfwd_zinv"] = 16.0  # (14 + 17 + 17) / 3
+        out["zfwd_rk"] = 9.0
+    return out
 
 
 def cpu_cores():
@@ -263,12 +278,13 @@ def bench_main(scl, a):
         copy_calibration(a.copy_calib)
 
     F = field_bytes(n)
+    kb = kind_bytes_f(kinds)
     kern = {}
     for k, (ms, cnt) in kinds.items():
         if cnt == 0:
             continue
         avg_s = ms / cnt * 1e-3
-        b = KIND_BYTES_F[k] * F
+        b = kb[k] * F
         kern[k] = {"ms_per_step": ms / a.steps, "launches": cnt, "avg_us": avg_s * 1e6,
                    "alg_bytes_per_launch": b, "achieved_GBps": (b / avg_s / 1e9) if b else None}
     dom = max(kern, key=lambda k: kern[k]["ms_per_step"])

@@ -208,10 +208,13 @@ int32_t scl_dns_diagnostics(scl_dns *d, scl_diag *out);
 /* Per-kernel timing of nsteps steps with HIP events on the plan's stream
  * around every launch (direct launches, also when use_graph is set: event
  * records captured into a graph cannot be timed).  kind_ms[k] /
- * kind_count[k] for k < SCL_NKIND, total_ms = event time of the whole run. */
+ * kind_count[k] for k < SCL_NKIND, total_ms = event time of the whole run.
+ * Within a step the z-forward of stages 1..3 also z-inverses the next stage
+ * state (SCL_K_ZFWD_ZINV; stage 4 and every stage of 1024-point z-lines:
+ * SCL_K_ZFWD, each stage then with its own SCL_K_ZINV). */
 enum {
   SCL_K_FORCE = 0, SCL_K_ZINV = 1, SCL_K_YINV = 2, SCL_K_XPASS = 3, SCL_K_YFWD = 4, SCL_K_ZFWD = 5,
-  SCL_K_HERM = 6, SCL_NKIND = 7
+  SCL_K_HERM = 6, SCL_K_ZFWD_ZINV = 7, SCL_NKIND = 8
 };
 int32_t scl_dns_step_timed(scl_dns *d, int32_t nsteps, double *kind_ms, int32_t *kind_count, double *total_ms);
 

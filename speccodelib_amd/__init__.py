@@ -24,7 +24,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "scl.h")
 
 SP, DP = 0, 1
 _STATUS = {0: "SCL_OK", -1: "SCL_EINVAL", -2: "SCL_ENOMEM", -3: "SCL_EHIP", -4: "SCL_ENOPLAN", -5: "SCL_ENONFINITE", -6: "SCL_ENODEV"}
-KIND_NAMES = ["force_coef", "zinv_vort", "y_inverse", "x_pass", "y_forward", "zfwd_rk", "herm_fix"]
+KIND_NAMES = ["force_coef", "zinv_vort", "y_inverse", "x_pass", "y_forward", "zfwd_rk", "herm_fix", "zfwd_zinv"]
 
 _lib = None
 
@@ -206,6 +206,7 @@ class Plan3D:
     """A plan (fftwplan.f90:7-15): grid, precision, stream, twiddles."""
 
     def __init__(self, nx, ny=None, nz=None, prec=SP):
+        self.h = None
         ny = ny or nx
         nz = nz or nx
         h = ctypes.c_void_p()
@@ -214,7 +215,7 @@ class Plan3D:
         self.shape = (nz, ny, nx // 2 + 1)
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             lib().scl_plan3d_destroy(self.h)
             self.h = None
 

@@ -529,7 +529,7 @@ struct Timer {
 // (the fused five-pass DNS RHS; nc = 3).
 template <class R>
 static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsParams<R>& rp, double eps, double eta,
-                        double fkmax, const cpx<R>* st, Timer* tm) {
+                        double fkmax, const cpx<R>* st, Timer* tm, bool primed = false, bool fuse = false) {
   const int64_t fs = d.fs;
   cpx<R>* V = d.V<R>();
   cpx<R>* DU = d.DU<R>();
@@ -541,22 +541,25 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
     tm->mark(SCL_K_FORCE);
     SCL_CHECK(Ops<R>::force_coef(*p, st, fs, fkmax, eps, eta, rp.forced, d.coef));
   }
-  tm->mark(SCL_K_ZINV);
-  SCL_CHECK(Ops<R>::zinv_vort(*p, st, fs, W, fs));
+  if (!primed) {
+    tm->mark(SCL_K_ZINV);
+    SCL_CHECK(Ops<R>::zinv_vort(*p, st, fs, W, fs));
+  }
   tm->mark(SCL_K_YINV);
   SCL_CHECK(Ops<R>::yinv_vort(*p, W, fs));
   tm->mark(SCL_K_XPASS);
   SCL_CHECK(Ops<R>::row_convec(*p, W, fs));
   tm->mark(SCL_K_YFWD);
   SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
-  tm->mark(SCL_K_ZFWD);
-  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp));
+  tm->mark(fuse ? SCL_K_ZFWD_ZINV : SCL_K_ZFWD);
+  SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp, fuse));
   // The kx = 0 Hermitian copy (symmetrize.f90:51-61) goes on N (mode 0),
   // on the collected stage states (s_out: the adjoint's tape), and on v once
   // per step.  Skipping it on du and the stage state of stages 1..3 changes
   // them only at rounding level: the kx = 0 plane of N is Hermitian up to
   // rounding (it is the transform of real products), and c2r ignores the
-  // anti-Hermitian part of the kx = 0 lines.
+  // anti-Hermitian part of the kx = 0 lines.  For the same reason the fused
+  // stage 4 may z-inverse v before its kx = 0 copy.
   if (mode >= 1 && mode < 4 && !d.s_out) {
     tm->mark(-1);
     return SCL_OK;
@@ -581,12 +584,26 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   return SCL_OK;
 }
 
+// Stage chaining within a step: the z-forward of stages 1..3 also
+// z-inverses the next stage state into W[0..4] (k_zfwd_rhs FUSED), so
+// stages 2..4 skip their z-inverse pass and the stage states are stored
+// only on the forced modes.  Each step starts from v (its own z-inverse
+// pass) and leaves no state in W, so a step's result does not depend on
+// how steps are grouped into calls.  Not for 1024-point z-lines (the fused
+// kernel spills there) nor when the stage states are collected (s_out).
+template <class R>
+static bool chain_stages(const scl_plan3d* p, const DevFields& d) {
+  return !d.s_out && p->g.nz <= 512;
+}
+
 template <class R>
 static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm) {
   const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  const bool chain = chain_stages<R>(p, d);
   for (int stage = 1; stage <= 4; ++stage) {
     const cpx<R>* st = stage == 1 ? d.V<R>() : d.S<R>();
-    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm));
+    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm, chain && stage > 1,
+                          chain && stage < 4));
   }
   return SCL_OK;
 }

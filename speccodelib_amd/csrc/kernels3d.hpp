@@ -166,31 +166,47 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
 // five transforms, stored to w[0..4] (u, then w_x, w_y) at stride ws.
 // w_z = i (kx u_y - ky u_x) needs no kz, so the y-inverse builds it from
 // the transformed u_x, u_y (k_yinv_vort) and one field less crosses HBM.
-template <class R, int N, int E, int CW>
-__device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds,
-                                  const cpx<R>* twl, int rt, int c, R kx, R ky, const int64_t (&off)[E],
-                                  const R (&kz)[E], cpx<R>* __restrict__ w, int64_t ws) {
+// In-field entry offsets fit in 32 bits (< 2^30 at 1024^3); each field has
+// its own base pointer.  PARK (1024-thread workgroups: one per CU at the
+// 128-VGPR cap, LDS to spare): Uz waits in park (E BS entries, each thread
+// reading back only its own) between its store and its two later uses.
+template <class R, int N, int E, int CW, bool PARK>
+__device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds, cpx<R>* park,
+                                  const cpx<R>* twl, int rt, int c, R kx, R ky, int nz, uint32_t ob, uint32_t rs,
+                                  cpx<R>* __restrict__ w, int64_t ws) {
+  constexpr int T = N / E, BS = CW * T;
   using L = Line<R, N, E, +1>;
+  // An empty asm that "changes" ob: the store addresses below are then
+  // recomputed here rather than kept live (and spilled) from the caller's
+  // loads of the same columns.
+  asm volatile("" : "+v"(ob));
+  auto off = [&](int m) { return ob + (uint32_t)(rt + T * m) * rs; };
+  auto kz = [&](int m) { return (R)kwave(rt + T * m, nz); };
+  const int slot = rt * CW + c;
   cpx<R> d[E];
   L::run(uz, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + 2 * ws] = uz[m];
+  for (int m = 0; m < E; ++m) {
+    (w + 2 * ws)[off(m)] = uz[m];
+    if constexpr (PARK) park[m * BS + slot] = uz[m];
+  }
+  auto Uz = [&](int m) { return PARK ? park[m * BS + slot] : uz[m]; };
 #pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = kz[m] * uy[m];
+  for (int m = 0; m < E; ++m) d[m] = kz(m) * uy[m];
   L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + 3 * ws] = mul_i(ky * uz[m] - d[m]);
+  for (int m = 0; m < E; ++m) (w + 3 * ws)[off(m)] = mul_i(ky * Uz(m) - d[m]);
   L::run(uy, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + ws] = uy[m];
+  for (int m = 0; m < E; ++m) (w + ws)[off(m)] = uy[m];
 #pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = kz[m] * ux[m];
+  for (int m = 0; m < E; ++m) d[m] = kz(m) * ux[m];
   L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m] + 4 * ws] = mul_i(d[m] - kx * uz[m]);
+  for (int m = 0; m < E; ++m) (w + 4 * ws)[off(m)] = mul_i(d[m] - kx * Uz(m));
   L::run(ux, lds, rt, twl, 1, ColIdx<CW>{c});
 #pragma unroll
-  for (int m = 0; m < E; ++m) w[off[m]] = ux[m];
+  for (int m = 0; m < E; ++m) w[off(m)] = ux[m];
 }
 
 // ZINV: z-inverse of the stage state s (3 fields at stride sstride) into
@@ -202,23 +218,24 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const
                                                                      const cpx<R>* __restrict__ tw, int tws) {
   using CC = ColCfg<R, N, CWZ, EZ>;
   constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  constexpr bool PARK = CC::BS >= 1024;
   __shared__ cpx<R> lds[lds_pad(N) * CW];
   __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
+  __shared__ cpx<R> park[PARK ? E * CC::BS : 1];
   for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
-  int64_t off[E];
-  R kz[E];
+  const uint32_t ob = (uint32_t)t.base, rs = (uint32_t)t.rs;
   cpx<R> ux[E], uy[E], uz[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    off[m] = t.base + (int64_t)(rt + T * m) * t.rs;
-    kz[m] = (R)kwave(rt + T * m, g.nz);
-    uz[m] = s[off[m] + 2 * sstride];
-    uy[m] = s[off[m] + sstride];
-    ux[m] = s[off[m]];
+    const uint32_t o = ob + (uint32_t)(rt + T * m) * rs;
+    uz[m] = s[o + 2 * sstride];
+    uy[m] = s[o + sstride];
+    ux[m] = s[o];
   }
-  zinv_store<R, N, E, CW>(ux, uy, uz, lds, twl, rt, c, (R)t.kx, (R)kwave(t.oth, g.ny), off, kz, w, wstride);
+  zinv_store<R, N, E, CW, PARK>(ux, uy, uz, lds, park, twl, rt, c, (R)t.kx, (R)kwave(t.oth, g.ny), g.nz, ob, rs, w,
+                                wstride);
 }
 
 // YINV: y-inverse of the five z-pass outputs plus w_z = i (kx Uy - ky Ux),
@@ -270,27 +287,40 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
 // One IF-RK4 stage of one component in the division-free form (see
 // k_zfwd_rhs): a = dt N, em1 = expm1(x), x = -rate dt/2, so E y = y + em1 y
 // and E^2 y = y + expm1(2x) y, each exact to the rounding of the result.
+// Returns the new stage state (MODE 1..3, stored to S only where store_s)
+// or the new v (MODE 4, always stored).
 template <int MODE, class R, class I>
-__device__ inline void stage_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
+__device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
+                                    bool store_s) {
+  cpx<R> out;
   if constexpr (MODE == 1) {
     const cpx<R> v = V[o];
     const cpx<R> u = v + ((R)1 / (R)6) * a, q = v + (R)0.5 * a;
     DU[o] = u + expm1_r((R)2 * x) * u;
-    S[o] = q + em1 * q;
+    out = q + em1 * q;
   } else if constexpr (MODE == 2 || MODE == 3) {
     const cpx<R> v = V[o];
     const cpx<R> q = ((R)1 / (R)3) * a;
     DU[o] = DU[o] + (q + em1 * q);
     const cpx<R> ev = v + em1 * v;
     if constexpr (MODE == 2) {
-      S[o] = ev + (R)0.5 * a;
+      out = ev + (R)0.5 * a;
     } else {
       const cpx<R> u = ev + a;
-      S[o] = u + em1 * u;
+      out = u + em1 * u;
     }
   } else {
-    V[o] = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
+    out = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
+    V[o] = out;
+    return out;
   }
+  if (store_s) S[o] = out;
+  return out;
+}
+
+template <int MODE, class R, class I>
+__device__ inline void stage_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
+  stage_keep<MODE, R>(a, o, V, DU, S, em1, x, cut, true);
 }
 
 // Parameters of the fused ZFWD epilogue.
@@ -318,7 +348,13 @@ struct RhsParams {
 //   MODE 3: du += E a/3;         S = E (E v + a)
 //   MODE 4: v = du + a/6, 0 where k2 >= kc2
 // so the last stage reads no v, and no stage divides by E.
-template <class R, int N, int MODE, int CWZ, int EZ = 0>
+//
+// FUSED (MODE 1..3): the new stage state stays in registers and its
+// z-inverse goes to W[0..4] for the next RHS (zinv_store), so that RHS
+// needs no z-inverse pass of its own; S is then stored only where the
+// forcing reads it (k2 < fk2).  Each workgroup reads all of its columns of
+// W before it writes them.
+template <class R, int N, int MODE, int CWZ, int EZ = 0, bool FUSED = false>
 __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V,
                                                                     cpx<R>* DU, cpx<R>* S, const cpx<R>* st,
                                                                     const double* __restrict__ coef, RhsParams<R> p,
@@ -336,18 +372,34 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
   const uint32_t ob = (uint32_t)t.base, rs = (uint32_t)t.rs;
   cpx<R>* const W1 = W + fstride;
   cpx<R>* const W2 = W + 2 * fstride;
+  // 1024-thread workgroups run one per CU at the 128-VGPR cap, which leaves
+  // LDS to spare: the transformed x component waits there (each thread
+  // reads back only its own entries) instead of holding 2E registers
+  // through the other two transforms, and z is loaded after x is parked.
+  constexpr bool PARK = CC::BS >= 1024;
+  __shared__ cpx<R> park[PARK ? E * CC::BS : 1];
   cpx<R> nx_[E], ny_[E], nz_[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const uint32_t o = ob + (uint32_t)(rt + T * m) * rs;
     nx_[m] = W[o];
     ny_[m] = W1[o];
-    nz_[m] = W2[o];
+    if constexpr (!PARK) nz_[m] = W2[o];
   }
   using L = Line<R, N, E, -1>;
   L::run(nx_, lds, rt, twl, 1, ColIdx<CW>{c});
+  if constexpr (PARK) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      park[m * CC::BS + threadIdx.x] = nx_[m];
+      nz_[m] = W2[ob + (uint32_t)(rt + T * m) * rs];
+    }
+  }
   L::run(ny_, lds, rt, twl, 1, ColIdx<CW>{c});
   L::run(nz_, lds, rt, twl, 1, ColIdx<CW>{c});
+  // the epilogue recomputes its offsets (not kept live from the loads)
+  uint32_t ob_ = ob;
+  asm volatile("" : "+v"(ob_));
 
   const int ikx = t.kx, iky = kwave(t.oth, g.ny);
   const R kx = (R)ikx, ky = (R)iky;
@@ -365,8 +417,8 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     const R kz = (R)ikz;
     const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
     const R k2 = ik2 == 0 ? tiny : (R)ik2;
-    const uint32_t o = ob + (uint32_t)z * rs;
-    cpx<R> a = nx_[m], b = ny_[m], d = nz_[m];
+    const uint32_t o = ob_ + (uint32_t)z * rs;
+    cpx<R> a = PARK ? park[m * CC::BS + threadIdx.x] : nx_[m], b = ny_[m], d = nz_[m];
     // forcing (force.f90:25-29 / force_heli.f90:44-52)
     if (p.forced && k2 < p.fk2) {
       const cpx<R> sx = st[o], sy = st[fstride + o], sz = st[2 * fstride + o];
@@ -410,10 +462,15 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R x = -p.nu * k2 * p.dt / (R)2;
       const R em1 = expm1_r(x);
       const bool cut = k2 >= p.kc2;
-      stage_update<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut);
-      stage_update<MODE, R>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut);
-      stage_update<MODE, R>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1, x, cut);
+      const bool store_s = !FUSED || (p.forced && k2 < p.fk2);
+      nx_[m] = stage_keep<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut, store_s);
+      ny_[m] = stage_keep<MODE, R>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut, store_s);
+      nz_[m] = stage_keep<MODE, R>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1, x, cut,
+                                   store_s);
     }
+  }
+  if constexpr (FUSED && MODE > 0) {
+    zinv_store<R, N, E, CW, PARK>(nx_, ny_, nz_, lds, park, twl, rt, c, kx, ky, g.nz, ob, rs, W, fstride);
   }
 }
 

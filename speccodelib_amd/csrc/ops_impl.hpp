@@ -124,8 +124,16 @@ static bool zcol16(const Geom& g) {
 #ifndef SCL_Z16_E
 #define SCL_Z16_E 0
 #endif
-template <int N, int CW>
+// Points per thread of the 1024-point FP32 z-lines (0: 16).  8 points make
+// 1024-thread workgroups at 4 waves per SIMD, but measured slower at 1024^3
+// than 16 points at 2 waves per SIMD (13.1 vs 12.6 ms z-inverse, 19.0 vs
+// 17.6 ms z-forward): one more LDS exchange per transform.
+#ifndef SCL_Z1024_E
+#define SCL_Z1024_E 0
+#endif
+template <class R, int N, int CW>
 constexpr int zpass_e() {
+  if (sizeof(R) == 4 && N == 1024 && CW == 8 && SCL_Z1024_E) return SCL_Z1024_E;
   return (CW == 16 && SCL_Z16_E && N >= SCL_Z16_E) ? SCL_Z16_E : 0;
 }
 
@@ -136,14 +144,14 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
     constexpr int N = decltype(NC)::value;
     auto go = [&](auto CWC) -> int32_t {
       constexpr int CW = decltype(CWC)::value;
-      constexpr int EZ = zpass_e<N, CW>();
+      constexpr int EZ = zpass_e<R, N, CW>();
       using CC = ColCfg<R, N, CW, EZ>;
       SCL_CHECK((check_cols<R, N, CW>(g)));
       hipLaunchKernelGGL((k_zinv_vort<R, N, CW, EZ>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s,
                          sstride, w, wstride, g, twp<R>(pl), pl.L / N);
       return check_launch("zinv_vort");
     };
-    if constexpr (sizeof(R) == 4) {
+    if constexpr (sizeof(R) == 4 && N <= 512) {
       if (zcol16<R>(g)) return go(IC<16>{});
     }
     return go(IC<8>{});
@@ -152,18 +160,27 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
 
 template <class R>
 int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
-                         const double* coef, const RhsParams<R>& p) {
+                         const double* coef, const RhsParams<R>& p, bool fused) {
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
     auto go = [&](auto CWC) -> int32_t {
       constexpr int CW = decltype(CWC)::value;
-      constexpr int EZ = zpass_e<N, CW>();
+      constexpr int EZ = zpass_e<R, N, CW>();
       using CC = ColCfg<R, N, CW, EZ>;
       SCL_CHECK((check_cols<R, N, CW>(g)));
       const dim3 grd(col_grid<CW, 2>(g)), blk(CC::BS);
       const cpx<R>* tw = twp<R>(pl);
       const int tws = pl.L / N;
+      if (fused) {
+        switch (mode) {
+          case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          default: set_error("fused z-forward: mode %d is not a stage 1..3", mode); return SCL_EINVAL;
+        }
+        return check_launch("zfwd_zinv");
+      }
       switch (mode) {
         case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
         case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
@@ -174,7 +191,7 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
       }
       return check_launch("zfwd_rhs");
     };
-    if constexpr (sizeof(R) == 4) {
+    if constexpr (sizeof(R) == 4 && N <= 512) {
       if (zcol16<R>(g)) return go(IC<16>{});
     }
     return go(IC<8>{});

@@ -130,6 +130,8 @@ template <> struct PlanOf<64, 8> { using L = RList<8, 8>; };
 template <> struct PlanOf<128, 8> { using L = RList<8, 4, 4>; };
 template <> struct PlanOf<256, 8> { using L = RList<8, 8, 4>; };
 template <> struct PlanOf<512, 8> { using L = RList<8, 8, 8>; };
+// 1024-point z-columns at 8 points per thread (1024-thread workgroups)
+template <> struct PlanOf<1024, 8> { using L = RList<8, 8, 4, 4>; };
 
 // One Stockham pass in registers (no data exchange).
 template <class R, int N, int E, int DIR, int NS, int RAD>
