@@ -589,11 +589,11 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
 // stages 2..4 skip their z-inverse pass and the stage states are stored
 // only on the forced modes.  Each step starts from v (its own z-inverse
 // pass) and leaves no state in W, so a step's result does not depend on
-// how steps are grouped into calls.  Not for 1024-point z-lines (the fused
-// kernel spills there) nor when the stage states are collected (s_out).
+// how steps are grouped into calls.  Not when the stage states are
+// collected (s_out: the adjoint's tape).
 template <class R>
-static bool chain_stages(const scl_plan3d* p, const DevFields& d) {
-  return !d.s_out && p->g.nz <= 512;
+static bool chain_stages(const scl_plan3d*, const DevFields& d) {
+  return !d.s_out;
 }
 
 template <class R>

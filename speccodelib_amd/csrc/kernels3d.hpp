@@ -167,9 +167,11 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
 // w_z = i (kx u_y - ky u_x) needs no kz, so the y-inverse builds it from
 // the transformed u_x, u_y (k_yinv_vort) and one field less crosses HBM.
 // In-field entry offsets fit in 32 bits (< 2^30 at 1024^3); each field has
-// its own base pointer.  PARK (1024-thread workgroups: one per CU at the
-// 128-VGPR cap, LDS to spare): Uz waits in park (E BS entries, each thread
-// reading back only its own) between its store and its two later uses.
+// its own base pointer.  PARK (1024-thread workgroups, one per CU at the
+// 128-VGPR cap, and 1024-point lines, one 512-thread workgroup per CU at
+// 2 waves per SIMD: LDS to spare in both): Uz waits in park (E BS entries,
+// each thread reading back only its own) between its store and its two
+// later uses.
 template <class R, int N, int E, int CW, bool PARK>
 __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds, cpx<R>* park,
                                   const cpx<R>* twl, int rt, int c, R kx, R ky, int nz, uint32_t ob, uint32_t rs,
@@ -218,7 +220,7 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const
                                                                      const cpx<R>* __restrict__ tw, int tws) {
   using CC = ColCfg<R, N, CWZ, EZ>;
   constexpr int CW = CC::CW, E = CC::E, T = CC::T;
-  constexpr bool PARK = CC::BS >= 1024;
+  constexpr bool PARK = CC::BS >= 1024 || (N >= 1024 && sizeof(R) == 4);
   __shared__ cpx<R> lds[lds_pad(N) * CW];
   __shared__ cpx<R> twl[N];  // this line length's twiddles; first read after a barrier
   __shared__ cpx<R> park[PARK ? E * CC::BS : 1];
@@ -372,11 +374,11 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
   const uint32_t ob = (uint32_t)t.base, rs = (uint32_t)t.rs;
   cpx<R>* const W1 = W + fstride;
   cpx<R>* const W2 = W + 2 * fstride;
-  // 1024-thread workgroups run one per CU at the 128-VGPR cap, which leaves
-  // LDS to spare: the transformed x component waits there (each thread
-  // reads back only its own entries) instead of holding 2E registers
+  // 1024-thread workgroups (and 1024-point lines) run one per CU, which
+  // leaves LDS to spare: the transformed x component waits there (each
+  // thread reads back only its own entries) instead of holding 2E registers
   // through the other two transforms, and z is loaded after x is parked.
-  constexpr bool PARK = CC::BS >= 1024;
+  constexpr bool PARK = CC::BS >= 1024 || (N >= 1024 && sizeof(R) == 4);
   __shared__ cpx<R> park[PARK ? E * CC::BS : 1];
   cpx<R> nx_[E], ny_[E], nz_[E];
 #pragma unroll
