@@ -240,10 +240,10 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const
                                 wstride);
 }
 
-// YINV: y-inverse of the five z-pass outputs plus w_z = i (kx Uy - ky Ux),
-// formed before the transform while ky is still a spectral index.  Reads 5
-// fields, writes 6, in place at stride fs.  blockIdx.y = 0 transforms u_x,
-// u_y and w_z; blockIdx.y = 1..3 transform fields 2..4 (u_z, w_x, w_y).
+// YINV (u_x, u_y and w_z): y-inverse of fields 0 and 1 (u_x, u_y) and of
+// w_z = i (kx Uy - ky Ux), formed before the transform while ky is still a
+// spectral index, into field 5; in place at stride fs.  Fields 2..4 (u_z,
+// w_x, w_y) take the plain column kernel.
 template <class R, int N>
 __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __restrict__ W, int64_t fs, Geom g,
                                                                 const cpx<R>* __restrict__ tw, int tws) {
@@ -264,16 +264,8 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
 #pragma unroll
     for (int m = 0; m < E; ++m) W[t.base + (int64_t)L::out_pos(m, rt) * t.rs + f * fs] = v[m];
   };
-  cpx<R> a[E];
-  if (blockIdx.y > 0) {
-    const int f = 1 + (int)blockIdx.y;
-    load(a, f);
-    L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
-    store(a, f);
-    return;
-  }
   const R kx = (R)t.kx;
-  cpx<R> b[E], d[E];
+  cpx<R> a[E], b[E], d[E];
   load(a, 0);
   load(b, 1);
 #pragma unroll

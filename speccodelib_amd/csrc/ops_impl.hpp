@@ -95,6 +95,10 @@ int32_t Ops<R>::colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int
   });
 }
 
+// y-inverse of the five z-pass outputs plus w_z: u_x, u_y, w_z in
+// k_yinv_vort (three fields live), then u_z, w_x, w_y in the plain column
+// kernel (one field live, twice the waves per CU; measured 1193 vs 1237 us
+// per RHS at 512^3 for one kernel doing all six).
 template <class R>
 int32_t Ops<R>::yinv_vort(const scl_plan3d& pl, C* W, int64_t fstride) {
   const Geom& g = pl.g;
@@ -102,9 +106,10 @@ int32_t Ops<R>::yinv_vort(const scl_plan3d& pl, C* W, int64_t fstride) {
     constexpr int N = decltype(NC)::value;
     using CC = ColCfg<R, N>;
     SCL_CHECK((check_cols<R, N>(g)));
-    hipLaunchKernelGGL((k_yinv_vort<R, N>), dim3(col_grid<CC::CW, 1>(g), 4), dim3(CC::BS), 0, pl.stream, W, fstride,
-                       g, twp<R>(pl), pl.L / N);
-    return check_launch("yinv_vort");
+    hipLaunchKernelGGL((k_yinv_vort<R, N>), dim3(col_grid<CC::CW, 1>(g)), dim3(CC::BS), 0, pl.stream, W, fstride, g,
+                       twp<R>(pl), pl.L / N);
+    SCL_CHECK(check_launch("yinv_vort"));
+    return colfft(pl, W + 2 * fstride, fstride, 3, 1, 1);
   });
 }
 
