@@ -209,8 +209,6 @@ int32_t group_step(const scl_ens2d& e, const Group& gr, hipStream_t st) {
   return SCL_OK;
 }
 
-// One step of every group.  Batched: the groups run on parallel branches
-// (fork/join through events, which graph capture turns into graph edges).
 // nsteps steps of every group.  Batched: the groups are independent, so each
 // runs its nsteps steps as one chain on its own branch (fork/join through
 // events, which graph capture turns into graph edges) and the branches
