@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_adj_hermT(cpx<R>* __restrict__ Y, int64
   } else {
     return;
   }
-  const int64_t od = ((int64_t)z * g.ny + y) * g.mx, os = ((int64_t)zp * g.ny + yp) * g.mx;
+  const int64_t od = g.row(y, z) * g.mx, os = g.row(yp, zp) * g.mx;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const cpx<R> t = Y[od + q * fs];

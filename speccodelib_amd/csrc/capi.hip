@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <cstdlib>
 #include <limits>
 #include <mutex>
 #include <vector>
@@ -230,9 +231,19 @@ extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t
     hipGetLastError();
     return set_error("no HIP device"), SCL_ENODEV;
   }
+  // Row order (Geom::zs): z-blocks of 8 rows for 1024-point z-lines, where
+  // the z-passes otherwise spend their time on address translation; plain
+  // [z][y] below.  SCL_ZBLOCK (1 or 8) overrides it, for tests of both
+  // orders at small sizes.
+  int zs = nz >= 1024 ? 3 : 0;
+  if (const char* zb = std::getenv("SCL_ZBLOCK")) {
+    const int b = std::atoi(zb);
+    if (b != 1 && b != 8) return set_error("SCL_ZBLOCK=%s: must be 1 or 8", zb), SCL_EINVAL;
+    zs = b == 8 ? 3 : 0;
+  }
   auto* p = new scl_plan3d();
   p->prec = prec;
-  p->g = make_geom(nx, ny, nz);
+  p->g = make_geom(nx, ny, nz, zs);
   hipGetDevice(&p->device);
   {
     int ncu = 0;

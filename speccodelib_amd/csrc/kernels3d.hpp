@@ -38,12 +38,33 @@ __device__ inline double expm1_r(double x) { return expm1(x); }
 // tail tiles hold the kx = nx/2 column for CW values of the other index.
 // ----------------------------------------------------------------------------
 struct Tile {
-  int64_t base;  // offset of this thread's column, row 0
-  int64_t rs;    // row stride
+  int64_t base;  // offset of this thread's column, index 0 along the axis
+  int64_t rs;    // y-pass: row stride; z-pass: stride of the rows of a z-block
+  int64_t bs;    // z-pass: stride of the z-blocks (y-pass: = rs)
+  int sh;        // z-pass: g.zs (y-pass: 0)
   int kx;        // x index of the column (mx for the kx = nx/2 plane)
   int oth;       // the other batch index: z for the y-pass, y for the z-pass
   bool ok;       // false for the unused lanes of a partial tail tile
+  // offset of index j along the axis
+  __device__ int64_t off(int j) const {
+    return base + (int64_t)(j >> sh) * bs + (int64_t)(j & ((1 << sh) - 1)) * rs;
+  }
 };
+
+// The z-pass addressing with 32-bit in-field offsets (< 2^30 at 1024^3) and
+// the z-block shift ZS (= g.zs) known at compile time: every field gets its
+// own base pointer, and a thread keeps one 32-bit offset per slot.
+template <int ZS>
+struct ZOff {
+  uint32_t ob, rs, bs;
+  __device__ uint32_t at(int j) const {
+    return ob + (uint32_t)(j >> ZS) * bs + (uint32_t)(j & ((1 << ZS) - 1)) * rs;
+  }
+};
+template <int ZS>
+__device__ inline ZOff<ZS> zoff(const Tile& t) {
+  return ZOff<ZS>{(uint32_t)t.base, (uint32_t)t.rs, (uint32_t)t.bs};
+}
 
 template <int CW, int AXIS>
 __device__ inline Tile col_tile(const Geom& g, int wg, int c) {
@@ -51,35 +72,40 @@ __device__ inline Tile col_tile(const Geom& g, int wg, int c) {
   Tile t;
   if constexpr (AXIS == 1) {
     const int nmain = xtn * g.nz;
+    t.sh = 0;
     if (wg < nmain) {
       const int xt = wg % xtn, z = wg / xtn;
-      t.base = (int64_t)z * g.ny * g.mx + xt * CW + c;
-      t.rs = g.mx;
+      t.base = g.row(0, z) * g.mx + xt * CW + c;
+      t.rs = (int64_t)g.mx << g.zs;
       t.kx = xt * CW + c;
       t.oth = z;
       t.ok = true;
     } else {
       const int z = (wg - nmain) * CW + c;
       t.ok = z < g.nz;
-      t.base = g.nmain + (int64_t)(t.ok ? z : 0) * g.ny;
-      t.rs = 1;
+      t.base = g.nmain + g.row(0, t.ok ? z : 0);
+      t.rs = (int64_t)1 << g.zs;
       t.kx = g.mx;
       t.oth = z;
     }
+    t.bs = t.rs;
   } else {
     const int nmain = xtn * g.ny;
+    t.sh = g.zs;
     if (wg < nmain) {
       const int xt = wg % xtn, y = wg / xtn;
-      t.base = (int64_t)y * g.mx + xt * CW + c;
-      t.rs = (int64_t)g.ny * g.mx;
+      t.base = g.row(y, 0) * g.mx + xt * CW + c;
+      t.rs = g.mx;
+      t.bs = ((int64_t)g.ny << g.zs) * g.mx;
       t.kx = xt * CW + c;
       t.oth = y;
       t.ok = true;
     } else {
       const int y = (wg - nmain) * CW + c;
       t.ok = y < g.ny;
-      t.base = g.nmain + (t.ok ? y : 0);
-      t.rs = g.ny;
+      t.base = g.nmain + g.row(t.ok ? y : 0, 0);
+      t.rs = 1;
+      t.bs = (int64_t)g.ny << g.zs;
       t.kx = g.mx;
       t.oth = y;
     }
@@ -152,11 +178,11 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
   const Tile t = col_tile<CW, AXIS>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
   cpx<R> v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = t.ok ? f[t.base + (int64_t)(rt + T * m) * t.rs] : mk<R>(0, 0);
+  for (int m = 0; m < E; ++m) v[m] = t.ok ? f[t.off(rt + T * m)] : mk<R>(0, 0);
   Line<R, N, E, DIR>::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
   if (!t.ok) return;
 #pragma unroll
-  for (int m = 0; m < E; ++m) f[t.base + (int64_t)Line<R, N, E, DIR>::out_pos(m, rt) * t.rs] = v[m];
+  for (int m = 0; m < E; ++m) f[t.off(Line<R, N, E, DIR>::out_pos(m, rt))] = v[m];
 }
 
 // z-inverse of a velocity column held in registers (slot m = index
@@ -166,23 +192,22 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
 // five transforms, stored to w[0..4] (u, then w_x, w_y) at stride ws.
 // w_z = i (kx u_y - ky u_x) needs no kz, so the y-inverse builds it from
 // the transformed u_x, u_y (k_yinv_vort) and one field less crosses HBM.
-// In-field entry offsets fit in 32 bits (< 2^30 at 1024^3); each field has
-// its own base pointer.  PARK (1024-thread workgroups, one per CU at the
+// Offsets are 32-bit (ZOff).  PARK (1024-thread workgroups, one per CU at the
 // 128-VGPR cap, and 1024-point lines, one 512-thread workgroup per CU at
 // 2 waves per SIMD: LDS to spare in both): Uz waits in park (E BS entries,
 // each thread reading back only its own) between its store and its two
 // later uses.
-template <class R, int N, int E, int CW, bool PARK>
+template <class R, int N, int E, int CW, bool PARK, int ZS>
 __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds, cpx<R>* park,
-                                  const cpx<R>* twl, int rt, int c, R kx, R ky, int nz, uint32_t ob, uint32_t rs,
+                                  const cpx<R>* twl, int rt, int c, R kx, R ky, int nz, ZOff<ZS> zo,
                                   cpx<R>* __restrict__ w, int64_t ws) {
   constexpr int T = N / E, BS = CW * T;
   using L = Line<R, N, E, +1>;
-  // An empty asm that "changes" ob: the store addresses below are then
-  // recomputed here rather than kept live (and spilled) from the caller's
-  // loads of the same columns.
-  asm volatile("" : "+v"(ob));
-  auto off = [&](int m) { return ob + (uint32_t)(rt + T * m) * rs; };
+  // An empty asm that "changes" the base offset: the store addresses below
+  // are then recomputed here rather than kept live (and spilled) from the
+  // caller's loads of the same columns.
+  asm volatile("" : "+v"(zo.ob));
+  auto off = [&](int m) { return zo.at(rt + T * m); };
   auto kz = [&](int m) { return (R)kwave(rt + T * m, nz); };
   const int slot = rt * CW + c;
   cpx<R> d[E];
@@ -213,7 +238,7 @@ __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)
 
 // ZINV: z-inverse of the stage state s (3 fields at stride sstride) into
 // w[0..4] (stride wstride), see zinv_store.
-template <class R, int N, int CWZ, int EZ = 0>
+template <class R, int N, int CWZ, int EZ = 0, int ZS = 0>
 __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const cpx<R>* __restrict__ s,
                                                                      int64_t sstride, cpx<R>* __restrict__ w,
                                                                      int64_t wstride, Geom g,
@@ -227,17 +252,17 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const
   for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
-  const uint32_t ob = (uint32_t)t.base, rs = (uint32_t)t.rs;
+  const ZOff<ZS> zo = zoff<ZS>(t);
   cpx<R> ux[E], uy[E], uz[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const uint32_t o = ob + (uint32_t)(rt + T * m) * rs;
+    const uint32_t o = zo.at(rt + T * m);
     uz[m] = s[o + 2 * sstride];
     uy[m] = s[o + sstride];
     ux[m] = s[o];
   }
-  zinv_store<R, N, E, CW, PARK>(ux, uy, uz, lds, park, twl, rt, c, (R)t.kx, (R)kwave(t.oth, g.ny), g.nz, ob, rs, w,
-                                wstride);
+  zinv_store<R, N, E, CW, PARK, ZS>(ux, uy, uz, lds, park, twl, rt, c, (R)t.kx, (R)kwave(t.oth, g.ny), g.nz, zo, w,
+                                    wstride);
 }
 
 // YINV (u_x, u_y and w_z): y-inverse of fields 0 and 1 (u_x, u_y) and of
@@ -257,12 +282,12 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
   using L = Line<R, N, E, +1>;
   auto load = [&](cpx<R>(&v)[E], int f) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = t.ok ? W[t.base + (int64_t)(rt + T * m) * t.rs + f * fs] : mk<R>(0, 0);
+    for (int m = 0; m < E; ++m) v[m] = t.ok ? W[t.off(rt + T * m) + f * fs] : mk<R>(0, 0);
   };
   auto store = [&](cpx<R>(&v)[E], int f) {
     if (!t.ok) return;
 #pragma unroll
-    for (int m = 0; m < E; ++m) W[t.base + (int64_t)L::out_pos(m, rt) * t.rs + f * fs] = v[m];
+    for (int m = 0; m < E; ++m) W[t.off(L::out_pos(m, rt)) + f * fs] = v[m];
   };
   const R kx = (R)t.kx;
   cpx<R> a[E], b[E], d[E];
@@ -348,7 +373,7 @@ struct RhsParams {
 // needs no z-inverse pass of its own; S is then stored only where the
 // forcing reads it (k2 < fk2).  Each workgroup reads all of its columns of
 // W before it writes them.
-template <class R, int N, int MODE, int CWZ, int EZ = 0, bool FUSED = false>
+template <class R, int N, int MODE, int CWZ, int EZ = 0, bool FUSED = false, int ZS = 0>
 __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V,
                                                                     cpx<R>* DU, cpx<R>* S, const cpx<R>* st,
                                                                     const double* __restrict__ coef, RhsParams<R> p,
@@ -360,10 +385,10 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
   for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
-  // Entry offsets within a field fit in 32 bits (< 2^30 at 1024^3); the
-  // three components go through their own base pointers, so a thread keeps
-  // one 32-bit offset per slot instead of 64-bit ones (which spilled).
-  const uint32_t ob = (uint32_t)t.base, rs = (uint32_t)t.rs;
+  // 32-bit in-field offsets: the three components go through their own
+  // base pointers, so a thread keeps one 32-bit offset per slot instead of
+  // 64-bit ones (which spilled).
+  const ZOff<ZS> zo = zoff<ZS>(t);
   cpx<R>* const W1 = W + fstride;
   cpx<R>* const W2 = W + 2 * fstride;
   // 1024-thread workgroups (and 1024-point lines) run one per CU, which
@@ -375,7 +400,7 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
   cpx<R> nx_[E], ny_[E], nz_[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const uint32_t o = ob + (uint32_t)(rt + T * m) * rs;
+    const uint32_t o = zo.at(rt + T * m);
     nx_[m] = W[o];
     ny_[m] = W1[o];
     if constexpr (!PARK) nz_[m] = W2[o];
@@ -386,14 +411,14 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       park[m * CC::BS + threadIdx.x] = nx_[m];
-      nz_[m] = W2[ob + (uint32_t)(rt + T * m) * rs];
+      nz_[m] = W2[zo.at(rt + T * m)];
     }
   }
   L::run(ny_, lds, rt, twl, 1, ColIdx<CW>{c});
   L::run(nz_, lds, rt, twl, 1, ColIdx<CW>{c});
   // the epilogue recomputes its offsets (not kept live from the loads)
-  uint32_t ob_ = ob;
-  asm volatile("" : "+v"(ob_));
+  ZOff<ZS> ze = zo;
+  asm volatile("" : "+v"(ze.ob));
 
   const int ikx = t.kx, iky = kwave(t.oth, g.ny);
   const R kx = (R)ikx, ky = (R)iky;
@@ -411,7 +436,7 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     const R kz = (R)ikz;
     const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
     const R k2 = ik2 == 0 ? tiny : (R)ik2;
-    const uint32_t o = ob_ + (uint32_t)z * rs;
+    const uint32_t o = ze.at(z);
     cpx<R> a = PARK ? park[m * CC::BS + threadIdx.x] : nx_[m], b = ny_[m], d = nz_[m];
     // forcing (force.f90:25-29 / force_heli.f90:44-52)
     if (p.forced && k2 < p.fk2) {
@@ -464,7 +489,7 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     }
   }
   if constexpr (FUSED && MODE > 0) {
-    zinv_store<R, N, E, CW, PARK>(nx_, ny_, nz_, lds, park, twl, rt, c, kx, ky, g.nz, ob, rs, W, fstride);
+    zinv_store<R, N, E, CW, PARK, ZS>(nx_, ny_, nz_, lds, park, twl, rt, c, kx, ky, g.nz, zo, W, fstride);
   }
 }
 
@@ -489,7 +514,7 @@ __global__ __launch_bounds__(256) void k_force_coef(const cpx<R>* __restrict__ s
     const int ik2 = kx * kx + ky * ky + kz * kz;
     const R k2 = ik2 == 0 ? tiny : (R)ik2;
     if (!(k2 < fk2)) continue;
-    const int64_t o = ix < g.mx ? ((int64_t)iz * g.ny + iy) * g.mx + ix : g.nmain + (int64_t)iz * g.ny + iy;
+    const int64_t o = ix < g.mx ? g.row(iy, iz) * g.mx + ix : g.nmain + g.row(iy, iz);
     const cpx<R> sx = st[o], sy = st[o + fstride], sz = st[o + 2 * fstride];
     const double wgt = ix == 0 ? 0.5 : 1.0;
     const double X[3][2] = {{sx.x, sx.y}, {sy.x, sy.y}, {sz.x, sz.y}};
@@ -561,7 +586,7 @@ __global__ __launch_bounds__(256) void k_herm_fix(FieldPtrs<R> fp, Geom g) {
   } else {
     return;
   }
-  const int64_t od = ((int64_t)z * g.ny + y) * g.mx, os = ((int64_t)zp * g.ny + yp) * g.mx;
+  const int64_t od = g.row(y, z) * g.mx, os = g.row(yp, zp) * g.mx;
   for (int f = 0; f < fp.n; ++f) fp.p[f][od] = conj(fp.p[f][os]);
 }
 
@@ -759,8 +784,9 @@ __global__ __launch_bounds__(256) void k_ref_to_split(const cpx<R>* __restrict__
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t n = g.nrow * (g.mx + 1);
   if (i >= n) return;
-  const int64_t row = i / (g.mx + 1);
+  const int64_t rr = i / (g.mx + 1);  // reference row z*ny + y
   const int x = (int)(i % (g.mx + 1));
+  const int64_t row = g.row((int)(rr % g.ny), (int)(rr / g.ny));
   if (x < g.mx) f[row * g.mx + x] = ref[i];
   else f[g.nmain + row] = ref[i];
 }
@@ -770,8 +796,9 @@ __global__ __launch_bounds__(256) void k_split_to_ref(const cpx<R>* __restrict__
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t n = g.nrow * (g.mx + 1);
   if (i >= n) return;
-  const int64_t row = i / (g.mx + 1);
+  const int64_t rr = i / (g.mx + 1);  // reference row z*ny + y
   const int x = (int)(i % (g.mx + 1));
+  const int64_t row = g.row((int)(rr % g.ny), (int)(rr / g.ny));
   ref[i] = x < g.mx ? f[row * g.mx + x] : f[g.nmain + row];
 }
 
@@ -785,8 +812,7 @@ __device__ inline void split_xyz(const Geom& g, int64_t i, int& x, int& y, int& 
     row = i - g.nmain;
     x = g.mx;
   }
-  y = (int)(row % g.ny);
-  z = (int)(row / g.ny);
+  g.yz(row, y, z);
 }
 
 // symmetrize.f90:41-48 zeroing (the Hermitian copy is k_herm_fix), and,

@@ -400,8 +400,9 @@ __global__ __launch_bounds__(256) void k_ref_to_split_real(const R* __restrict__
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t n = g.nrow * (g.mx + 1);
   if (i >= n) return;
-  const int64_t row = i / (g.mx + 1);
+  const int64_t rr = i / (g.mx + 1);  // reference row z*ny + y
   const int x = (int)(i % (g.mx + 1));
+  const int64_t row = g.row((int)(rr % g.ny), (int)(rr / g.ny));
   if (x < g.mx) f[row * g.mx + x] = ref[i];
   else f[g.nmain + row] = ref[i];
 }

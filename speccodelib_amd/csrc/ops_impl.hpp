@@ -142,6 +142,12 @@ constexpr int zpass_e() {
   return (CW == 16 && SCL_Z16_E && N >= SCL_Z16_E) ? SCL_Z16_E : 0;
 }
 
+// The fused z-pass kernels are compiled for the row orders zs = 0 and 3.
+inline int32_t check_zs(const Geom& g) {
+  if (g.zs != 0 && g.zs != 3) return set_error("row order zs = %d: the z-pass kernels take 0 or 3", g.zs), SCL_EINVAL;
+  return SCL_OK;
+}
+
 template <class R>
 int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* w, int64_t wstride) {
   const Geom& g = pl.g;
@@ -152,8 +158,13 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
       constexpr int EZ = zpass_e<R, N, CW>();
       using CC = ColCfg<R, N, CW, EZ>;
       SCL_CHECK((check_cols<R, N, CW>(g)));
-      hipLaunchKernelGGL((k_zinv_vort<R, N, CW, EZ>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s,
-                         sstride, w, wstride, g, twp<R>(pl), pl.L / N);
+      SCL_CHECK(check_zs(g));
+      if (g.zs == 3)
+        hipLaunchKernelGGL((k_zinv_vort<R, N, CW, EZ, 3>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s,
+                           sstride, w, wstride, g, twp<R>(pl), pl.L / N);
+      else
+        hipLaunchKernelGGL((k_zinv_vort<R, N, CW, EZ, 0>), dim3(col_grid<CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s,
+                           sstride, w, wstride, g, twp<R>(pl), pl.L / N);
       return check_launch("zinv_vort");
     };
     if constexpr (sizeof(R) == 4 && N <= 512) {
@@ -174,24 +185,35 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
       constexpr int EZ = zpass_e<R, N, CW>();
       using CC = ColCfg<R, N, CW, EZ>;
       SCL_CHECK((check_cols<R, N, CW>(g)));
+      SCL_CHECK(check_zs(g));
       const dim3 grd(col_grid<CW, 2>(g)), blk(CC::BS);
       const cpx<R>* tw = twp<R>(pl);
       const int tws = pl.L / N;
+      auto launch = [&](auto MC, auto FC, auto ZC) {
+        constexpr int MODE = decltype(MC)::value, ZS = decltype(ZC)::value;
+        constexpr bool FUSED = decltype(FC)::value != 0;
+        hipLaunchKernelGGL((k_zfwd_rhs<R, N, MODE, CW, EZ, FUSED, ZS>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st,
+                           coef, p, g, tw, tws);
+      };
+      auto by_zs = [&](auto MC, auto FC) {
+        if (g.zs == 3) launch(MC, FC, IC<3>{});
+        else launch(MC, FC, IC<0>{});
+      };
       if (fused) {
         switch (mode) {
-          case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-          case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-          case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW, EZ, true>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+          case 1: by_zs(IC<1>{}, IC<1>{}); break;
+          case 2: by_zs(IC<2>{}, IC<1>{}); break;
+          case 3: by_zs(IC<3>{}, IC<1>{}); break;
           default: set_error("fused z-forward: mode %d is not a stage 1..3", mode); return SCL_EINVAL;
         }
         return check_launch("zfwd_zinv");
       }
       switch (mode) {
-        case 0: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 0, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 1: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 1, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 2: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 2, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 3: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 3, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
-        case 4: hipLaunchKernelGGL((k_zfwd_rhs<R, N, 4, CW, EZ>), grd, blk, 0, pl.stream, W, fstride, V, DU, S, st, coef, p, g, tw, tws); break;
+        case 0: by_zs(IC<0>{}, IC<0>{}); break;
+        case 1: by_zs(IC<1>{}, IC<0>{}); break;
+        case 2: by_zs(IC<2>{}, IC<0>{}); break;
+        case 3: by_zs(IC<3>{}, IC<0>{}); break;
+        case 4: by_zs(IC<4>{}, IC<0>{}); break;
         default: set_error("bad mode %d", mode); return SCL_EINVAL;
       }
       return check_launch("zfwd_rhs");

@@ -63,21 +63,37 @@ __host__ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b) {
   return a * b;
 }
 
-// Geometry of a 3D plan, shared by all kernels.
+// Geometry of a 3D plan, shared by all kernels.  Row (y, z) holds the mx
+// main entries of that (y, z) and, at nmain + row, its kx = nx/2 entry.
+// Rows are ordered [z >> zs][y][z & (2^zs - 1)]: with zs = 3, the 8
+// consecutive z-rows of a column lie in one mx * 8-entry span, so a z-column
+// of 1024 rows touches 128 such spans instead of 1024 rows 4 MB apart (each
+// a separate address translation); zs = 0 is plain [z][y].
 struct Geom {
   int nx, ny, nz;  // physical grid
   int mx;          // nx/2: complex entries per main row
+  int zs;          // log2 of the z-block of the row order
   int64_t nrow;    // ny*nz rows
   int64_t nmain;   // mx*ny*nz complex entries in main
   int64_t ntot;    // nmain + ny*nz (main + nyq)
+  __host__ __device__ int64_t row(int y, int z) const {
+    return ((((int64_t)(z >> zs)) * ny + y) << zs) + (z & ((1 << zs) - 1));
+  }
+  __host__ __device__ void yz(int64_t r, int& y, int& z) const {
+    const int64_t blk = (int64_t)ny << zs;
+    const int64_t zb = r / blk, rem = r - zb * blk;
+    y = (int)(rem >> zs);
+    z = (int)((zb << zs) + (rem & ((1 << zs) - 1)));
+  }
 };
 
-inline Geom make_geom(int nx, int ny, int nz) {
+inline Geom make_geom(int nx, int ny, int nz, int zs = 0) {
   Geom g;
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
   g.mx = nx / 2;
+  g.zs = zs;
   g.nrow = (int64_t)ny * nz;
   g.nmain = (int64_t)g.mx * g.nrow;
   g.ntot = g.nmain + g.nrow;
