@@ -102,7 +102,8 @@ def cpu_baseline(n, steps, scl):
 # nu = 1e-5 k^8 hyperviscosity, linear damping 0.1, forcing on 4 <= |k| <= 5,
 # dt = 5e-4, 1000 IF-RK4 steps.  Per RHS 15 F2 (y-inverse 1 in + 3 out,
 # x-pass 3 + 2, y-forward 2 + state and stage I/O), 58 F2 per step with
-# F2 = 8 (n/2 + 1) n bytes per member.
+# F2 = 8 (n/2 + 1) n bytes per member (the model the roofline fraction is
+# priced against; the stage chain moves 51 F2 per step).
 ENS2D_GROUPS = [(256, 192), (512, 48), (1024, 16)]
 ENS2D_KW = dict(dt=5e-4, c_hyper=1e-5, n_hyper=8, damp=0.1, fkmin=4.0, fkmax=5.0, eps=0.01)
 ENS2D_STEP_F2 = 58.0

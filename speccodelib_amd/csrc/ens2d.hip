@@ -171,8 +171,11 @@ int64_t row_grid_size(const scl_ens2d& e, const Group& gr) {
 }
 
 // One RHS evaluation (mode 0) or IF-RK4 stage (modes 1..4) of a group.
+// primed: W[0..2] already hold the y-inverse of the stage vorticity (the
+// previous stage was fused); fuse: the y-forward also y-inverses the new
+// stage vorticity into W[0..2] (k_yfwd2d FUSED), for the next stage.
 template <class R>
-int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st) {
+int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st, bool primed = false, bool fuse = false) {
   using C = cpx<R>;
   C* W = gr.field<R>(F_W);
   C* WS = gr.field<R>(F_WS);
@@ -182,12 +185,15 @@ int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st) {
   const C* tw = reinterpret_cast<const C*>(gr.tw);
   const int64_t fs = gr.fs();
   const Rhs2D<R> rp = rhs_params<R>(e, gr.n);
-  SCL_CHECK(with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
-    constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
-    hipLaunchKernelGGL((k_yinv2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
-                       sin, W, fs, gr.g, tw, 1);
-    return check_launch("yinv2d");
-  }));
+  if (fuse && mode < 1) return set_error("fused 2D stage: mode %d is not a stage 1..4", mode), SCL_EINVAL;
+  if (!primed) {
+    SCL_CHECK(with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
+      constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
+      hipLaunchKernelGGL((k_yinv2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
+                         sin, W, fs, gr.g, tw, 1);
+      return check_launch("yinv2d");
+    }));
+  }
   SCL_CHECK(with_row2d<R>(gr.n / 2, [&](auto MC) -> int32_t {
     constexpr int M = decltype(MC)::value;
     const R scale = (R)1 / (R)((double)gr.n * gr.n);
@@ -197,15 +203,26 @@ int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st) {
   }));
   return with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
     constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
-    hipLaunchKernelGGL((k_yfwd2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
-                       W, fs, WS, ACC, S, sin, mode, rp, gr.g, tw, 1);
+    const dim3 grd(col_grid<CW, 1>(gr.g)), blk(Col2D<R, N, E, CW>::BS);
+    if (fuse)
+      hipLaunchKernelGGL((k_yfwd2d<R, N, E, CW, true>), grd, blk, 0, st, W, fs, WS, ACC, S, sin, mode, rp, gr.g, tw, 1);
+    else
+      hipLaunchKernelGGL((k_yfwd2d<R, N, E, CW, false>), grd, blk, 0, st, W, fs, WS, ACC, S, sin, mode, rp, gr.g, tw,
+                         1);
     return check_launch("yfwd2d");
   });
 }
 
+// nsteps steps of a group as one stage chain: the first stage runs its own
+// y-inverse, every later one gets it from the previous stage's fused
+// y-forward, and the last stage is not fused (the chain leaves no state in
+// W).  The kx = 0 column is made Hermitian in the same workgroup, so the
+// chain gives the same results as separate stages.
 template <class R>
-int32_t group_step(const scl_ens2d& e, const Group& gr, hipStream_t st) {
-  for (int mode = 1; mode <= 4; ++mode) SCL_CHECK(stage<R>(e, gr, mode, st));
+int32_t group_steps(const scl_ens2d& e, const Group& gr, hipStream_t st, int nsteps) {
+  for (int s = 0; s < nsteps; ++s)
+    for (int mode = 1; mode <= 4; ++mode)
+      SCL_CHECK(stage<R>(e, gr, mode, st, s > 0 || mode > 1, s < nsteps - 1 || mode < 4));
   return SCL_OK;
 }
 
@@ -219,8 +236,7 @@ int32_t launch_steps(scl_ens2d& e, int nsteps) {
   for (auto& gr : e.groups) {
     hipStream_t st = fork ? gr.stream : e.stream;
     if (fork) SCL_HIP(hipStreamWaitEvent(st, e.fork, 0));
-    for (int s = 0; s < nsteps; ++s)
-      SCL_CHECK(e.prec == SCL_SP ? group_step<float>(e, gr, st) : group_step<double>(e, gr, st));
+    SCL_CHECK(e.prec == SCL_SP ? group_steps<float>(e, gr, st, nsteps) : group_steps<double>(e, gr, st, nsteps));
     if (fork) {
       SCL_HIP(hipEventRecord(gr.done, st));
       SCL_HIP(hipStreamWaitEvent(e.stream, gr.done, 0));

@@ -73,6 +73,45 @@ struct Rhs2D {
   Rate2D<R> rate;
 };
 
+// y-inverse of u = i ky w/k^2, v = -i kx w/k^2 and w (spectral_dns_2d.f90:
+// 161-162,166-168, y part) from the vorticity column o held in registers
+// (slot m = index rt + T m along y, as every transform leaves it), stored
+// to the work fields w[0..2] at stride fstride.  One field is formed,
+// transformed and stored at a time, so at most two columns of E points are
+// live per thread.
+template <class R, int N, int E, int CW>
+__device__ inline void yinv2d_store(cpx<R> (&o)[E], cpx<R>* lds, const cpx<R>* twl, int rt, int c, const Tile& t,
+                                    cpx<R>* __restrict__ w, int64_t fstride) {
+  constexpr int T = N / E;
+  const R kx = (R)t.kx;
+  const R tiny = std::numeric_limits<R>::min();
+  using L = Line<R, N, E, +1>;
+  auto ik2r = [&](int m) {
+    const int iky = kwave(rt + T * m, N);
+    const int ik2 = t.kx * t.kx + iky * iky;
+    return ik2 == 0 ? tiny : (R)ik2;
+  };
+  auto store = [&](cpx<R>(&v)[E], int f) {
+    if (!t.ok) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[t.base + (int64_t)L::out_pos(m, rt) * t.rs + f * fstride] = v[m];
+  };
+  cpx<R> a[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const R ky = (R)kwave(rt + T * m, N);
+    a[m] = mk<R>(-(ky * o[m].y) / ik2r(m), (ky * o[m].x) / ik2r(m));
+  }
+  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(a, 0);
+#pragma unroll
+  for (int m = 0; m < E; ++m) a[m] = mk<R>((kx * o[m].y) / ik2r(m), -(kx * o[m].x) / ik2r(m));
+  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(a, 1);
+  L::run(o, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(o, 2);
+}
+
 // YINV.  s: the stage vorticity; w: three work fields (u, v, w) at stride fstride.
 template <class R, int N, int E, int CW>
 __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yinv2d(const cpx<R>* __restrict__ s,
@@ -85,40 +124,10 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yinv2d(const cpx<R
   for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
   const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
   const Tile t = col_tile<CW, 1>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
-  const R kx = (R)t.kx;
-  const R tiny = std::numeric_limits<R>::min();
-  using L = Line<R, N, E, +1>;
-  // one field is formed, transformed and stored at a time, so at most two
-  // columns of E points are live per thread
-  cpx<R> o[E], a[E];
-  R ik2r[E];
+  cpx<R> o[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int y = rt + T * m;
-    o[m] = t.ok ? s[t.base + (int64_t)y * t.rs] : mk<R>(0, 0);
-    const int iky = kwave(y, N);
-    const int ik2 = t.kx * t.kx + iky * iky;
-    ik2r[m] = ik2 == 0 ? tiny : (R)ik2;
-  }
-  auto store = [&](cpx<R>(&v)[E], int f) {
-    if (!t.ok) return;
-#pragma unroll
-    for (int m = 0; m < E; ++m) w[t.base + (int64_t)L::out_pos(m, rt) * t.rs + f * fstride] = v[m];
-  };
-  // u = i ky w/k^2, v = -i kx w/k^2 (spectral_dns_2d.f90:161-162)
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const R ky = (R)kwave(rt + T * m, N);
-    a[m] = mk<R>(-(ky * o[m].y) / ik2r[m], (ky * o[m].x) / ik2r[m]);
-  }
-  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(a, 0);
-#pragma unroll
-  for (int m = 0; m < E; ++m) a[m] = mk<R>((kx * o[m].y) / ik2r[m], -(kx * o[m].x) / ik2r[m]);
-  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(a, 1);
-  L::run(o, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(o, 2);
+  for (int m = 0; m < E; ++m) o[m] = t.ok ? s[t.base + (int64_t)(rt + T * m) * t.rs] : mk<R>(0, 0);
+  yinv2d_store<R, N, E, CW>(o, lds, twl, rt, c, t, w, fstride);
 }
 
 // XPASS: per row, c2r of u, v, w; u w and v w times scale; r2c of both
@@ -186,7 +195,12 @@ __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_adv2d(cpx<R>* W, int
 // which is dns-deform.f90:164-237's scheme multiplied through by E, E and
 // E^2 (the reference form divides by E, which underflows under k^8
 // hyperviscosity).  E x is applied as x + expm1(.) x.
-template <class R, int N, int E, int CW>
+// FUSED (modes 1..4): the new stage vorticity (S, or w in mode 4) stays in
+// registers, its kx = 0 column made Hermitian as it is stored, and its
+// y-inverse (yinv2d_store) goes to W[0..2] for the next RHS, which then
+// needs no y-inverse pass; S is stored only in the forced band, where the
+// forcing reads it.
+template <class R, int N, int E, int CW, bool FUSED = false>
 __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, int64_t fstride, cpx<R>* WS,
                                                                    cpx<R>* ACC, cpx<R>* S, const cpx<R>* st, int mode,
                                                                    Rhs2D<R> p, Geom g,
@@ -221,6 +235,12 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
   // That equals symmetrizing N first: every output is a real-linear function
   // of N and of inputs that already have this symmetry.
   if (kx0_tile) __syncthreads();  // the transforms' LDS reads are done
+  auto in_band = [&](int y) {
+    const int iky = kwave(y, N);
+    const int ik2 = ikx * ikx + iky * iky;
+    const R k2 = ik2 == 0 ? tiny : (R)ik2;
+    return p.forced && k2 >= p.fk2min && k2 <= p.fk2max;
+  };
   auto put = [&](uint32_t y, cpx<R> v1, cpx<R> v2) {
     const int64_t o = t.base + (int64_t)y * t.rs;
     if (mode == 0) {
@@ -229,9 +249,10 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
       WS[o] = v1;
     } else {
       ACC[o] = v1;
-      S[o] = v2;
+      if (!FUSED || in_band(y)) S[o] = v2;
     }
   };
+  cpx<R> ns[FUSED ? E : 1];  // the new stage vorticity (FUSED)
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int y = L::out_pos(m, rt);
@@ -277,6 +298,7 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
       }
     }
     if (t.ok) put(y, o1, o2);
+    if constexpr (FUSED) ns[m] = mode == 4 ? o1 : o2;
     if (kx0_tile && c == 0) {
       lds[y] = o1;
       lds[N + y] = o2;
@@ -288,9 +310,17 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int y = L::out_pos(m, rt);
-        if (y > N / 2) put(y, conj(lds[N - y]), conj(lds[2 * N - y]));
+        if (y > N / 2) {
+          const cpx<R> v1 = conj(lds[N - y]), v2 = conj(lds[2 * N - y]);
+          put(y, v1, v2);
+          if constexpr (FUSED) ns[m] = mode == 4 ? v1 : v2;
+        }
       }
     }
+  }
+  if constexpr (FUSED) {
+    if (kx0_tile) __syncthreads();  // the staged kx = 0 column is read
+    yinv2d_store<R, N, E, CW>(ns, lds, twl, rt, c, t, W, fstride);
   }
 }
 
