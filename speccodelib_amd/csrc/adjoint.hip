@@ -274,15 +274,15 @@ int32_t by_prec_a(const scl_adjoint* a, F&& f) {
 
 extern "C" int32_t scl_adjoint_destroy(scl_adjoint* a) {
   if (!a) return SCL_OK;
-  if (a->plan && a->plan->stream) hipStreamSynchronize(a->plan->stream);
+  if (a->plan && a->plan->stream) (void)hipStreamSynchronize(a->plan->stream);
   for (void* ptr : {a->fields, a->work, a->tape, a->ckpts, a->staging}) {
-    if (ptr) hipFree(ptr);
+    if (ptr) (void)hipFree(ptr);
   }
-  if (a->coef) hipFree(a->coef);
-  if (a->part) hipFree(a->part);
-  if (a->acc) hipFree(a->acc);
-  if (a->e0) hipEventDestroy(a->e0);
-  if (a->e1) hipEventDestroy(a->e1);
+  if (a->coef) (void)hipFree(a->coef);
+  if (a->part) (void)hipFree(a->part);
+  if (a->acc) (void)hipFree(a->acc);
+  if (a->e0) (void)hipEventDestroy(a->e0);
+  if (a->e1) (void)hipEventDestroy(a->e1);
   delete a;
   return SCL_OK;
 }
@@ -307,7 +307,7 @@ extern "C" int32_t scl_adjoint_create(scl_plan3d* p, const scl_dns_params* prm, 
   const size_t fb = (size_t)fs * p->esize();
   auto alloc = [&](void** ptr, size_t bytes) -> bool {
     if (hipMalloc(ptr, bytes) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();
       set_error("device allocation of %zu bytes failed", bytes);
       return false;
     }

@@ -81,14 +81,14 @@ void twiddles(int L, std::vector<double>& re, std::vector<double>& im) {
 int32_t plan_scratch(scl_plan3d* p, size_t nfields, void** out) {
   if (!p->scratch || p->scratch_fields < nfields) {
     if (p->scratch) {
-      hipStreamSynchronize(p->stream);
-      hipFree(p->scratch);
+      (void)hipStreamSynchronize(p->stream);
+      (void)hipFree(p->scratch);
       p->scratch = nullptr;
       p->scratch_fields = 0;
     }
     const size_t bytes = nfields * (size_t)p->fstride() * p->esize() + p->ref_bytes();
     if (hipMalloc(&p->scratch, bytes) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();
       set_error("device allocation of %zu bytes failed", bytes);
       return SCL_ENOMEM;
     }
@@ -101,13 +101,13 @@ int32_t plan_scratch(scl_plan3d* p, size_t nfields, void** out) {
 int32_t plan_diag(scl_plan3d* p, size_t bytes, void** out) {
   if (p->diag_bytes < bytes) {
     if (p->diag) {
-      hipStreamSynchronize(p->stream);
-      hipFree(p->diag);
+      (void)hipStreamSynchronize(p->stream);
+      (void)hipFree(p->diag);
       p->diag = nullptr;
       p->diag_bytes = 0;
     }
     if (hipMalloc(&p->diag, bytes) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();
       set_error("device allocation of %zu bytes failed", bytes);
       return SCL_ENOMEM;
     }
@@ -228,7 +228,7 @@ extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t
     return set_error("grid (%d,%d,%d): each size must be a power of two in [16,%d]", nx, ny, nz, hi), SCL_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-    hipGetLastError();
+    (void)hipGetLastError();
     return set_error("no HIP device"), SCL_ENODEV;
   }
   // Row order (Geom::zs): z-blocks of 8 rows for 1024-point z-lines, where
@@ -244,7 +244,7 @@ extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t
   auto* p = new scl_plan3d();
   p->prec = prec;
   p->g = make_geom(nx, ny, nz, zs);
-  hipGetDevice(&p->device);
+  (void)hipGetDevice(&p->device);
   {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess && ncu > 0)
@@ -278,20 +278,20 @@ extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t
 
 extern "C" int32_t scl_plan3d_destroy(scl_plan3d* p) {
   if (!p) return SCL_OK;
-  if (p->stream) hipStreamSynchronize(p->stream);
-  if (p->tw) hipFree(p->tw);
-  if (p->coef) hipFree(p->coef);
-  if (p->scratch) hipFree(p->scratch);
-  if (p->diag) hipFree(p->diag);
-  if (p->stream && p->own_stream) hipStreamDestroy(p->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->tw) (void)hipFree(p->tw);
+  if (p->coef) (void)hipFree(p->coef);
+  if (p->scratch) (void)hipFree(p->scratch);
+  if (p->diag) (void)hipFree(p->diag);
+  if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return SCL_OK;
 }
 
 extern "C" int32_t scl_plan3d_set_stream(scl_plan3d* p, void* stream) {
   if (!p) return set_error("null plan"), SCL_EINVAL;
-  hipStreamSynchronize(p->stream);
-  if (p->own_stream) hipStreamDestroy(p->stream);
+  (void)hipStreamSynchronize(p->stream);
+  if (p->own_stream) (void)hipStreamDestroy(p->stream);
   if (stream) {
     p->stream = (hipStream_t)stream;
     p->own_stream = false;
@@ -529,8 +529,8 @@ struct Timer {
   void mark(int kind) {
     if (!ev) return;
     hipEvent_t e;
-    hipEventCreate(&e);
-    hipEventRecord(e, s);
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, s);
     ev->push_back(e);
     kinds->push_back(kind);
   }
@@ -735,13 +735,13 @@ static DevFields dev_fields(const scl_dns* d) {
 }
 
 static void free_dns(scl_dns* d) {
-  if (d->exec) hipGraphExecDestroy(d->exec);
-  if (d->graph) hipGraphDestroy(d->graph);
-  if (d->fields) hipFree(d->fields);
-  if (d->dyn) hipFree(d->dyn);
-  if (d->gt) hipFree(d->gt);
-  if (d->part) hipFree(d->part);
-  if (d->coef) hipFree(d->coef);
+  if (d->exec) (void)hipGraphExecDestroy(d->exec);
+  if (d->graph) (void)hipGraphDestroy(d->graph);
+  if (d->fields) (void)hipFree(d->fields);
+  if (d->dyn) (void)hipFree(d->dyn);
+  if (d->gt) (void)hipFree(d->gt);
+  if (d->part) (void)hipFree(d->part);
+  if (d->coef) (void)hipFree(d->coef);
   delete d;
 }
 
@@ -766,7 +766,7 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
   const size_t fb = (size_t)p->fstride() * p->esize();
   const size_t bytes = (size_t)(3 * d->nc + d->nw) * fb;
   auto fail = [&](size_t b) {
-    hipGetLastError();
+    (void)hipGetLastError();
     free_dns(d);
     set_error("device allocation of %zu bytes for the solver failed", b);
     return SCL_ENOMEM;
@@ -779,9 +779,13 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
     if (hipMalloc((void**)&d->part, (2 * ExtOps<float>::DIAG_BLOCKS_DYN + 8) * sizeof(double)) != hipSuccess)
       return fail(8192);
   }
-  hipMemsetAsync(d->fields, 0, bytes, p->stream);
   double c0[8] = {0, 0, 0, 0, 0, prm->cs2, 0, 0};
-  hipMemcpyAsync(d->coef, c0, sizeof(c0), hipMemcpyHostToDevice, p->stream);
+  if (hipMemsetAsync(d->fields, 0, bytes, p->stream) != hipSuccess ||
+      hipMemcpyAsync(d->coef, c0, sizeof(c0), hipMemcpyHostToDevice, p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    free_dns(d);
+    return set_error("solver initialisation failed"), SCL_EHIP;
+  }
   if (prm->sgs == 2) {
     const double kc = 2.0 * (p->g.nx / 2) / 3.0;
     const double kt = prm->k_test > 0 ? prm->k_test : kc / 2.0;
@@ -804,7 +808,7 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
 
 extern "C" int32_t scl_dns_destroy(scl_dns* d) {
   if (!d) return SCL_OK;
-  hipStreamSynchronize(d->plan->stream);
+  (void)hipStreamSynchronize(d->plan->stream);
   free_dns(d);
   return SCL_OK;
 }
@@ -894,7 +898,7 @@ static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(p->stream, &gr);
       if (st != SCL_OK) {
-        if (gr) hipGraphDestroy(gr);
+        if (gr) (void)hipGraphDestroy(gr);
         return st;
       }
       if (e != hipSuccess) return set_error("graph capture failed: %s", hipGetErrorString(e)), SCL_EHIP;
@@ -942,7 +946,7 @@ extern "C" int32_t scl_dns_step_timed(scl_dns* d, int32_t nsteps, double* kind_m
     using R = decltype(zero);
     return dns_launch_steps<R>(d, nsteps, &tm);
   });
-  hipStreamSynchronize(p->stream);
+  (void)hipStreamSynchronize(p->stream);
   if (st == SCL_OK) {
     for (int k = 0; k < SCL_NKIND; ++k) {
       if (kind_ms) kind_ms[k] = 0.0;
@@ -951,17 +955,17 @@ extern "C" int32_t scl_dns_step_timed(scl_dns* d, int32_t nsteps, double* kind_m
     for (size_t i = 0; i + 1 < ev.size(); ++i) {
       if (kinds[i] < 0) continue;
       float ms = 0.f;
-      hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
       if (kind_ms) kind_ms[kinds[i]] += ms;
       if (kind_count) kind_count[kinds[i]] += 1;
     }
     if (total_ms && ev.size() >= 2) {
       float ms = 0.f;
-      hipEventElapsedTime(&ms, ev.front(), ev.back());
+      (void)hipEventElapsedTime(&ms, ev.front(), ev.back());
       *total_ms = ms;
     }
   }
-  for (auto e : ev) hipEventDestroy(e);
+  for (auto e : ev) (void)hipEventDestroy(e);
   return st;
 }
 

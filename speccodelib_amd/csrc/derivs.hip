@@ -206,7 +206,7 @@ extern "C" int32_t scl_ifrk4_integrate(int32_t prec, int64_t n, void* y, const v
     using C = cpx<R>;
     C* work = nullptr;
     if (hipMalloc(&work, 3 * n * sizeof(C)) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();
       return set_error("device allocation of %zu bytes failed", (size_t)(3 * n * sizeof(C))), SCL_ENOMEM;
     }
     C* fv = work;
@@ -237,8 +237,8 @@ extern "C" int32_t scl_ifrk4_integrate(int32_t prec, int64_t n, void* y, const v
       }
       t += dt;
     }
-    hipDeviceSynchronize();
-    hipFree(work);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(work);
     return st;
   });
 }

@@ -247,8 +247,8 @@ int32_t launch_steps(scl_ens2d& e, int nsteps) {
 
 void drop_graph(scl_ens2d& e) {
   for (int i = 0; i < 2; ++i) {
-    if (e.exec[i]) hipGraphExecDestroy(e.exec[i]);
-    if (e.graph[i]) hipGraphDestroy(e.graph[i]);
+    if (e.exec[i]) (void)hipGraphExecDestroy(e.exec[i]);
+    if (e.graph[i]) (void)hipGraphDestroy(e.graph[i]);
     e.exec[i] = nullptr;
     e.graph[i] = nullptr;
   }
@@ -262,7 +262,7 @@ int32_t graph_exec(scl_ens2d& e, int i, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e.stream, &g);
     if (st != SCL_OK) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return st;
     }
     SCL_HIP(ce);
@@ -382,12 +382,12 @@ int32_t upload_member_f64(const scl_ens2d& e, const Group& gr, int b, const std:
 
 void destroy_groups(scl_ens2d& e) {
   for (auto& gr : e.groups) {
-    if (gr.base) hipFree(gr.base);
-    if (gr.stream) hipStreamDestroy(gr.stream);
-    if (gr.done) hipEventDestroy(gr.done);
+    if (gr.base) (void)hipFree(gr.base);
+    if (gr.stream) (void)hipStreamDestroy(gr.stream);
+    if (gr.done) (void)hipEventDestroy(gr.done);
   }
   e.groups.clear();
-  for (auto& t : e.tw) hipFree(t.second);
+  for (auto& t : e.tw) (void)hipFree(t.second);
   e.tw.clear();
 }
 
@@ -422,7 +422,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
   if (params->n_hyper < 0 || params->n_hypo < 0) return set_error("negative viscosity order"), SCL_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-    hipGetLastError();
+    (void)hipGetLastError();
     return set_error("no HIP device"), SCL_ENODEV;
   }
   auto* e = new scl_ens2d();
@@ -437,7 +437,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
   };
   {
     int dev = 0, ncu = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
       e->ncu = ncu;
   }
@@ -473,7 +473,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
       gr.cw = 8;
       const size_t bytes = (size_t)NFIELDS * gr.fs() * es;
       if (hipMalloc(&gr.base, bytes) != hipSuccess) {
-        hipGetLastError();
+        (void)hipGetLastError();
         e->groups.push_back(gr);
         return set_error("device allocation of %zu bytes failed", bytes), fail(SCL_ENOMEM);
       }
@@ -500,14 +500,14 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
 
 extern "C" int32_t scl_ens2d_destroy(scl_ens2d* e) {
   if (!e) return SCL_OK;
-  if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop_graph(*e);
   destroy_groups(*e);
-  if (e->diag) hipFree(e->diag);
-  if (e->fork) hipEventDestroy(e->fork);
-  if (e->t0) hipEventDestroy(e->t0);
-  if (e->t1) hipEventDestroy(e->t1);
-  if (e->stream) hipStreamDestroy(e->stream);
+  if (e->diag) (void)hipFree(e->diag);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->t0) (void)hipEventDestroy(e->t0);
+  if (e->t1) (void)hipEventDestroy(e->t1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return SCL_OK;
 }
