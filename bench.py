@@ -242,6 +242,39 @@ def bench_heli1024(scl, steps, n=1024):
     }
 
 
+# 256^3 FP32 static-Smagorinsky LES with a passive scalar (spectral_les.f90,
+# spectral_dns_scalar.f90): C_s = 0.17, Delta = 2 pi/256, nu = 1e-4,
+# dt = 0.001, forcing on |k| <= 2.5, scalar with unit mean gradient and
+# Sc = 0.7 started from zero; seeded k^4 exp(-2 (k/4)^2) field.
+def bench_les256(scl, steps, n=256):
+    plan = scl.Plan3D(n, n, n, scl.SP)
+    d = scl.DNS(plan, nu=1e-4, dt=0.001, forced=1, fkmax=2.5, eps=0.1, sgs=1, cs2=0.17**2, delta=2 * np.pi / n,
+                scalar=True, grad=1.0, schmidt=0.7, use_graph=True)
+    d.initialize(1, 4.0, 1.0)
+    d.set_scalar(np.zeros((n, n, n // 2 + 1), np.complex64))
+    d.step(2)
+    d.sync()
+    t0 = time.perf_counter()
+    d.step(steps)
+    d.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    dg = d.diagnostics()
+    d.close()
+    plan.close()
+    print(f"[bench] LES + scalar {n}^3: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+    return {
+        "workload": f"static Smagorinsky LES + passive scalar {n}^3 FP32, C_s=0.17, nu=1e-4, dt=0.001, Sc=0.7, "
+                    "forcing |k|<=2.5",
+        "steps": steps,
+        "ms_per_step": ms,
+        "steps_per_s": 1e3 / ms,
+        "ns_per_point_step": ms * 1e6 / n**3,
+        "energy": dg["energy"],
+        "scalar_variance": dg["scalar_variance"],
+        "nonfinite": dg["nonfinite"],
+    }
+
+
 def load_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
     if os.path.exists(path):
@@ -341,6 +374,7 @@ def main():
     ap.add_argument("--no-ens2d", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--heli1024", type=int, default=0, help="also time this many steps of the 1024^3 helical DNS")
+    ap.add_argument("--les256", type=int, default=0, help="also time this many steps of the 256^3 LES + scalar")
     ap.add_argument("--no-main", action="store_true", help="skip the 512^3 DNS (profiling runs of the others)")
     ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -364,6 +398,8 @@ def main():
         out["adjoint_forward_ratio"] = out["adjoint"]["ratio"]
     if a.heli1024:
         out["heli1024"] = bench_heli1024(scl, a.heli1024)
+    if a.les256:
+        out["les256"] = bench_les256(scl, a.les256)
     if not a.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, scl)
