@@ -145,10 +145,19 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode) {
   const int64_t nh = (int64_t)(g.nz / 2 - 1) * g.ny + g.ny / 2 - 1;
   hipLaunchKernelGGL((k_adj_hermT<R>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, p->stream, Y, fs, g);
   SCL_CHECK(check_launch("adj_hermT"));
-  hipLaunchKernelGGL((k_adj_prep<R>), dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, p->stream, s, Y, WA, fs,
-                     g, a->L);
-  SCL_CHECK(check_launch("adj_prep"));
-  SCL_CHECK(Ops<R>::c2r3d(*p, WA, fs, nin));
+  // prep fused into the z-inverse (1024-point z-lines: separate passes).
+  // The combine stays a separate pass: fused with the z-forward it ran at
+  // one wave per SIMD and made the 128^3 backward slower (568 vs 536 ms).
+  if (g.nz < 1024) {
+    SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L));
+    SCL_CHECK(Ops<R>::colfft(*p, WA, fs, nin, 1, +1));
+    SCL_CHECK(Ops<R>::row_c2r(*p, WA, fs, nin));
+  } else {
+    hipLaunchKernelGGL((k_adj_prep<R>), dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, p->stream, s, Y, WA, fs,
+                       g, a->L);
+    SCL_CHECK(check_launch("adj_prep"));
+    SCL_CHECK(Ops<R>::c2r3d(*p, WA, fs, nin));
+  }
   const double delta = a->p.delta > 0 ? a->p.delta : 2.0 * M_PI / g.nx;
   const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
   const R d2 = (R)(delta * delta);
@@ -162,8 +171,8 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode) {
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, p->stream, a->part, (int)nb, a->acc);
     SCL_CHECK(check_launch("adj_sum"));
   }
-  SCL_CHECK(Ops<R>::r2c3d(*p, WA, fs, a->L.nout()));
   const AdjParams<R> ap = adj_params<R>(a);
+  SCL_CHECK(Ops<R>::r2c3d(*p, WA, fs, a->L.nout()));
   const dim3 ga((unsigned)((g.ntot + 255) / 256));
   switch (mode) {
     case 0: hipLaunchKernelGGL((k_adj_combine<R, 0>), ga, dim3(256), 0, p->stream, WA, fs, LAM, VB, Y, g, a->L, ap); break;

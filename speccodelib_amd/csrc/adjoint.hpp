@@ -277,6 +277,117 @@ __global__ __launch_bounds__(256) void k_adj_combine(const cpx<R>* __restrict__ 
   }
 }
 
+// ZINV of the VJP inputs: k_adj_prep fused with the z-pass of their inverse
+// transform.  Per column tile the stage state s is loaded and u, w, S_ij
+// are formed in registers, z-transformed and stored; then Y is loaded and
+// ybar (zeroed on the symmetrize planes, times the inverse weight) and
+// taubar_ij are formed and transformed the same way.  The y- and x-passes
+// follow as for any field.
+template <class R, int N>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_adj_zinv(const cpx<R>* __restrict__ s,
+                                                               const cpx<R>* __restrict__ Y, cpx<R>* __restrict__ W,
+                                                               int64_t fs, Geom g, AdjLayout L,
+                                                               const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  using Ln = Line<R, N, E, +1>;
+  const int ikx = t.kx, iky = kwave(t.oth, g.ny);
+  const R kx = (R)ikx, ky = (R)iky;
+  auto kzi = [&](int m) { return kwave(rt + T * m, g.nz); };
+  auto kz = [&](int m) { return (R)kzi(m); };
+  const ZOffR zo = zoffr(t);
+  auto store = [&](cpx<R>(&d)[E], int q) {
+    if (!t.ok) return;
+    ZOffR zs = zo;
+    asm volatile("" : "+v"(zs.ob));  // recomputed here, not kept live from the loads
+    cpx<R>* const wq = W + q * fs;
+#pragma unroll
+    for (int m = 0; m < E; ++m) wq[zs.at(Ln::out_pos(m, rt))] = d[m];
+  };
+  cpx<R> v[3][E], d[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const uint32_t o = zo.at(rt + T * m);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q][m] = t.ok ? (s + q * fs)[o] : mk<R>(0, 0);
+  }
+  // w = i k x u
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(ky * v[2][m] - kz(m) * v[1][m]);
+  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(d, 3);
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[0][m] - kx * v[2][m]);
+  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(d, 4);
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(kx * v[1][m] - ky * v[0][m]);
+  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(d, 5);
+  if (L.les) {
+    constexpr int P[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = P[q][0], j = P[q][1];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const R k[3] = {kx, ky, kz(m)};
+        d[m] = i == j ? mul_i(k[i] * v[i][m]) : mul_i((R)0.5 * (k[i] * v[j][m] + k[j] * v[i][m]));
+      }
+      Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+      store(d, 6 + q);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    Ln::run(v[q], lds, rt, twl, 1, ColIdx<CW>{c});
+    store(v[q], q);
+  }
+  // ybar = Z(Y) times the inverse weights; v's registers are free again
+  const R wi = (ikx == 0 || ikx == g.mx) ? (R)1 : (R)0.5;
+  cpx<R>(&yb)[3][E] = v;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int ikz = kzi(m);
+    const bool zero = sym_zero_mode(g, ikx, t.oth, rt + T * m, ikx * ikx + iky * iky + ikz * ikz);
+    const uint32_t o = zo.at(rt + T * m);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) yb[q][m] = (zero || !t.ok) ? mk<R>(0, 0) : wi * (Y + q * fs)[o];
+  }
+  if (L.les) {
+    // taubar from ybar (adjoint of N_i -= i k_j tau_ij), order 11 12 13 22 23 33
+    const int tq = L.tau_in();
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const R kzm = kz(m);
+        switch (q) {
+          case 0: d[m] = mul_i(kx * yb[0][m]); break;
+          case 1: d[m] = mul_i(ky * yb[0][m] + kx * yb[1][m]); break;
+          case 2: d[m] = mul_i(kzm * yb[0][m] + kx * yb[2][m]); break;
+          case 3: d[m] = mul_i(ky * yb[1][m]); break;
+          case 4: d[m] = mul_i(kzm * yb[1][m] + ky * yb[2][m]); break;
+          default: d[m] = mul_i(kzm * yb[2][m]); break;
+        }
+      }
+      Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+      store(d, tq + q);
+    }
+  }
+  const int yo = L.y_in();
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    Ln::run(yb[q], lds, rt, twl, 1, ColIdx<CW>{c});
+    store(yb[q], yo + q);
+  }
+}
+
 // Transpose of the kx = 0 Hermitian copy (symmetrize.f90:51-61) on Y:
 // for each target t = (0, y, z) (kz < 0, or kz = 0 and ky < 0) with source
 // s = (0, -y, -z): Y[s] += conj(Y[t]), Y[t] = 0.  Sources are never targets.

@@ -65,6 +65,15 @@ template <int ZS>
 __device__ inline ZOff<ZS> zoff(const Tile& t) {
   return ZOff<ZS>{(uint32_t)t.base, (uint32_t)t.rs, (uint32_t)t.bs};
 }
+// The same with the shift known at run time (kernels off the step's path).
+struct ZOffR {
+  uint32_t ob, rs, bs;
+  int sh;
+  __device__ uint32_t at(int j) const {
+    return ob + (uint32_t)(j >> sh) * bs + (uint32_t)(j & ((1 << sh) - 1)) * rs;
+  }
+};
+__device__ inline ZOffR zoffr(const Tile& t) { return ZOffR{(uint32_t)t.base, (uint32_t)t.rs, (uint32_t)t.bs, t.sh}; }
 
 template <int CW, int AXIS>
 __device__ inline Tile col_tile(const Geom& g, int wg, int c) {

@@ -217,6 +217,240 @@ __global__ __launch_bounds__(256) void k_ext_combine(cpx<R>* W, int64_t fs, cpx<
   }
 }
 
+// ZINV of the extended inputs: k_ext_prep fused with the z-pass of their
+// inverse transform.  Per column tile the stage state u (st, st+fs, st+2fs)
+// is loaded once; each input field (w = i k x u, S_ij, then u itself and
+// phi at st+3fs) is formed in registers, z-transformed and stored to W at
+// its ExtLayout index; the y- and x-passes follow as for any field.
+template <class R, int N>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zinv(const cpx<R>* __restrict__ st,
+                                                               cpx<R>* __restrict__ W, int64_t fs, Geom g,
+                                                               ExtLayout L, const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  using Ln = Line<R, N, E, +1>;
+  const R kx = (R)t.kx, ky = (R)kwave(t.oth, g.ny);
+  auto kz = [&](int m) { return (R)kwave(rt + T * m, g.nz); };
+  // 32-bit in-field offsets (< 2^30 at 1024^3), one base pointer per field
+  const ZOffR zo = zoffr(t);
+  cpx<R> v[3][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const uint32_t o = zo.at(rt + T * m);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q][m] = t.ok ? (st + q * fs)[o] : mk<R>(0, 0);
+  }
+  auto store = [&](cpx<R>(&d)[E], int q) {
+    if (!t.ok) return;
+    ZOffR zs = zo;
+    asm volatile("" : "+v"(zs.ob));  // recomputed here, not kept live from the loads
+    cpx<R>* const wq = W + q * fs;
+#pragma unroll
+    for (int m = 0; m < E; ++m) wq[zs.at(Ln::out_pos(m, rt))] = d[m];
+  };
+  cpx<R> d[E];
+  // w = i k x u (as k_ext_prep)
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(ky * v[2][m] - kz(m) * v[1][m]);
+  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(d, 3);
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[0][m] - kx * v[2][m]);
+  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(d, 4);
+#pragma unroll
+  for (int m = 0; m < E; ++m) d[m] = mul_i(kx * v[1][m] - ky * v[0][m]);
+  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  store(d, 5);
+  if (L.les) {
+    // S_ij = (i/2)(k_i u_j + k_j u_i), order 11 12 13 22 23 33 (strain_of)
+    constexpr int P[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = P[q][0], j = P[q][1];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const R k[3] = {kx, ky, kz(m)};
+        d[m] = i == j ? mul_i(k[i] * v[i][m]) : mul_i((R)0.5 * (k[i] * v[j][m] + k[j] * v[i][m]));
+      }
+      Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+      store(d, L.s_in() + q);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    Ln::run(v[q], lds, rt, twl, 1, ColIdx<CW>{c});
+    store(v[q], q);
+  }
+  if (L.scalar) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = t.ok ? (st + 3 * fs)[zo.at(rt + T * m)] : mk<R>(0, 0);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, L.phi_in());
+  }
+}
+
+// ZFWD of the extended outputs: the z-pass of their forward transform fused
+// with k_ext_combine.  Each output field (u x w, tau_ij, u_j phi; already
+// x- and y-transformed) is z-transformed in turn and folded into the three
+// velocity components N and the scalar rate R_phi held in registers
+// (N_i -= i k_j tau_ij, R_phi -= i k.F, term by term), then the combine's
+// epilogue runs per entry: forcing, symmetrize's zeroing, projection, the
+// scalar source -G u_z, the optional cut, and the IF-RK4 stage (MODE 1..4)
+// or N, R_phi into W[0..3] (MODE 0).  Every field's columns are read before
+// W is written.
+template <class R, int N, int MODE>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zfwd(cpx<R>* W, int64_t fs, cpx<R>* V, cpx<R>* DU,
+                                                               cpx<R>* S, const cpx<R>* st,
+                                                               const double* __restrict__ coef, ExtParams<R> p,
+                                                               Geom g, ExtLayout L, const cpx<R>* __restrict__ tw,
+                                                               int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 2>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  using Ln = Line<R, N, E, -1>;
+  const int ikx = t.kx, iky = kwave(t.oth, g.ny);
+  const R kx = (R)ikx, ky = (R)iky;
+  auto kz = [&](int m) { return (R)kwave(Ln::out_pos(m, rt), g.nz); };
+  const ZOffR zo = zoffr(t);
+  auto load = [&](cpx<R>(&d)[E], int q) {
+    ZOffR zl = zo;
+    asm volatile("" : "+v"(zl.ob));  // offsets recomputed per field, not kept live
+    const cpx<R>* const wq = W + q * fs;
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = t.ok ? wq[zl.at(rt + T * m)] : mk<R>(0, 0);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  };
+  cpx<R> a[E], b[E], dz[E], rp[E], f[E];
+  load(a, 0);
+  load(b, 1);
+  load(dz, 2);
+  if (L.les) {
+    // tau order 11 12 13 22 23 33: N_i -= i k_j tau_ij
+    const int t0 = L.tau_out();
+    load(f, t0);  // tau_11
+#pragma unroll
+    for (int m = 0; m < E; ++m) a[m] = a[m] - mul_i(kx * f[m]);
+    load(f, t0 + 1);  // tau_12
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      a[m] = a[m] - mul_i(ky * f[m]);
+      b[m] = b[m] - mul_i(kx * f[m]);
+    }
+    load(f, t0 + 2);  // tau_13
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      a[m] = a[m] - mul_i(kz(m) * f[m]);
+      dz[m] = dz[m] - mul_i(kx * f[m]);
+    }
+    load(f, t0 + 3);  // tau_22
+#pragma unroll
+    for (int m = 0; m < E; ++m) b[m] = b[m] - mul_i(ky * f[m]);
+    load(f, t0 + 4);  // tau_23
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      b[m] = b[m] - mul_i(kz(m) * f[m]);
+      dz[m] = dz[m] - mul_i(ky * f[m]);
+    }
+    load(f, t0 + 5);  // tau_33
+#pragma unroll
+    for (int m = 0; m < E; ++m) dz[m] = dz[m] - mul_i(kz(m) * f[m]);
+  }
+  if (L.scalar) {
+    // R_phi = -sign i k.F (convecphi_dns.f90:26-47's form, physical sign
+    // for the solver)
+    const int f0 = L.flux_out();
+    const R sg = (R)(-p.sign_phi);
+    load(f, f0);
+#pragma unroll
+    for (int m = 0; m < E; ++m) rp[m] = sg * mul_i(kx * f[m]);
+    load(f, f0 + 1);
+#pragma unroll
+    for (int m = 0; m < E; ++m) rp[m] = rp[m] + sg * mul_i(ky * f[m]);
+    load(f, f0 + 2);
+#pragma unroll
+    for (int m = 0; m < E; ++m) rp[m] = rp[m] + sg * mul_i(kz(m) * f[m]);
+  }
+  if (!t.ok) return;
+  const R tiny = std::numeric_limits<R>::min();
+  R fa = 0, fb = 0;
+  if (p.r.forced) {
+    fa = (R)coef[0];
+    fb = (R)coef[1];
+  }
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int z = Ln::out_pos(m, rt);
+    const int ikz = kwave(z, g.nz);
+    const R kzm = (R)ikz;
+    const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
+    const R k2 = ik2 == 0 ? tiny : (R)ik2;
+    const int64_t o = zo.at(z);
+    cpx<R> ax = a[m], by = b[m], cz = dz[m];
+    if (p.r.forced && k2 < p.r.fk2) {
+      const cpx<R> sx = st[o], sy = st[o + fs], sz = st[o + 2 * fs];
+      ax = ax + fa * sx;
+      by = by + fa * sy;
+      cz = cz + fa * sz;
+      if (p.r.forced == 2) {
+        ax = ax + fb * mul_i(ky * sz - kzm * sy);
+        by = by + fb * mul_i(kzm * sx - kx * sz);
+        cz = cz + fb * mul_i(kx * sy - ky * sx);
+      }
+    }
+    const bool zero = t.kx == g.mx || t.oth == g.ny / 2 || z == g.nz / 2 || ik2 == 0;
+    if (zero) {
+      ax = mk<R>(0, 0);
+      by = ax;
+      cz = ax;
+    }
+    {
+      const R den = kx * kx + ky * ky + kzm * kzm + tiny;
+      const cpx<R> q = mk<R>((kx * ax.x + ky * by.x + kzm * cz.x) / den, (kx * ax.y + ky * by.y + kzm * cz.y) / den);
+      ax = ax - kx * q;
+      by = by - ky * q;
+      cz = cz - kzm * q;
+    }
+    cpx<R> r = mk<R>(0, 0);
+    if (L.scalar) {
+      r = rp[m] - p.grad * st[o + 2 * fs];
+      if (zero) r = mk<R>(0, 0);
+    }
+    if (p.r.dealias && k2 >= p.r.kc2) {
+      ax = mk<R>(0, 0);
+      by = ax;
+      cz = ax;
+      r = ax;
+    }
+    if constexpr (MODE == 0) {
+      W[o] = ax;
+      W[o + fs] = by;
+      W[o + 2 * fs] = cz;
+      if (L.scalar) W[o + 3 * fs] = r;
+    } else {
+      const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
+      const R em1 = expm1_r(xv);
+      const bool cut = k2 >= p.r.kc2;
+      stage_update<MODE, R>(p.r.dt * ax, o, V, DU, S, em1, xv, cut);
+      stage_update<MODE, R>(p.r.dt * by, o + fs, V, DU, S, em1, xv, cut);
+      stage_update<MODE, R>(p.r.dt * cz, o + 2 * fs, V, DU, S, em1, xv, cut);
+      if (L.scalar) {
+        const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
+        stage_update<MODE, R>(p.r.dt * r, o + 3 * fs, V, DU, S, expm1_r(xp), xp, cut);
+      }
+    }
+  }
+}
+
 // --------------------------------------------------------------------------
 // Dynamic coefficient (dynsmag.f90:27-194) building blocks.
 // --------------------------------------------------------------------------

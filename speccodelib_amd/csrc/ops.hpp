@@ -8,6 +8,11 @@
 
 namespace scl {
 
+struct ExtLayout;
+template <class R>
+struct ExtParams;
+struct AdjLayout;
+
 template <class R>
 struct Ops {
   using C = cpx<R>;
@@ -18,6 +23,14 @@ struct Ops {
   // the next RHS (k_zfwd_rhs FUSED)
   static int32_t zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
                           const double* coef, const RhsParams<R>& p, bool fused = false);
+  // the LES / scalar RHS's z-passes fused with its prep and combine
+  // (k_ext_zinv, k_ext_zfwd); st and W share the field stride fs
+  static int32_t ext_zinv(const scl_plan3d& pl, const C* st, C* W, int64_t fs, const ExtLayout& L);
+  static int32_t ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
+                          const double* coef, const ExtParams<R>& ep, const ExtLayout& L);
+  // the adjoint VJP's z-inverse fused with its prep (k_adj_zinv); all
+  // fields share the stride fs
+  static int32_t adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L);
   static int32_t force_coef(const scl_plan3d& pl, const C* st, int64_t fstride, double fkmax, double eps, double eta,
                             int type, double* coef);
   static int32_t herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp);

@@ -17,12 +17,29 @@ struct ExtOps {
   static int32_t rhs(const scl_plan3d& pl, int mode, ExtLayout L, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
                      const double* coef, const ExtParams<R>& ep, R delta2) {
     const Geom& g = pl.g;
-    hipLaunchKernelGGL((k_ext_prep<R>), grid_all(g), dim3(256), 0, pl.stream, st, W, fs, g, L);
-    SCL_CHECK(check_launch("ext_prep"));
-    SCL_CHECK(Ops<R>::c2r3d(pl, W, fs, L.nin()));
+    // 1024-point z-lines keep the separate prep and combine: the fused
+    // z-passes spill there
+    const bool fused = g.nz < 1024;
+    if (fused) {
+      // prep fused into the z-inverse; then the y- and x-passes
+      SCL_CHECK(Ops<R>::ext_zinv(pl, st, W, fs, L));
+      SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 1, +1));
+      SCL_CHECK(Ops<R>::row_c2r(pl, W, fs, L.nin()));
+    } else {
+      hipLaunchKernelGGL((k_ext_prep<R>), grid_all(g), dim3(256), 0, pl.stream, st, W, fs, g, L);
+      SCL_CHECK(check_launch("ext_prep"));
+      SCL_CHECK(Ops<R>::c2r3d(pl, W, fs, L.nin()));
+    }
     const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
     hipLaunchKernelGGL((k_ext_phys<R>), grid_main(g), dim3(256), 0, pl.stream, W, fs, g, L, scale, delta2, coef + 5);
     SCL_CHECK(check_launch("ext_phys"));
+    if (fused) {
+      // the x- and y-passes of the forward transform; the z-pass fused with
+      // the combine
+      SCL_CHECK(Ops<R>::row_r2c(pl, W, fs, L.nout()));
+      SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 1, -1));
+      return Ops<R>::ext_zfwd(pl, mode, W, fs, V, DU, S, st, coef, ep, L);
+    }
     SCL_CHECK(Ops<R>::r2c3d(pl, W, fs, L.nout()));
     switch (mode) {
       case 0: hipLaunchKernelGGL((k_ext_combine<R, 0>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
