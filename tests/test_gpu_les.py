@@ -77,6 +77,31 @@ def test_convecphi_dns(prec):
     assert relerr(got, want) < (1e-11 if prec == scl.DP else 3e-6)
 
 
+def test_les_scalar_rhs_fp64():
+    """scl_dns_rhs of the LES + scalar solver (the RHS mode of its fused
+    z-passes) against the oracle's N - nu k^2 y (scalar: nu/Sc), 1e-11."""
+    n, nu, sc = 32, 1e-3, 0.7
+    delta = 2 * np.pi / n
+    print("case les+scalar rhs fp64", flush=True)
+    prm = les.LESParams(nx=n, nu=nu, dt=0.002, forced=True, fkmax=2.5, eps=0.1, cs2=0.0289, delta=delta,
+                        scalar=True, G=1.0, Sc=sc)
+    o = les.LESScalar(prm)
+    p = scl.Plan3D(n, n, n, scl.DP)
+    s = scl.DNS(p, nu=nu, dt=0.002, forced=1, fkmax=2.5, eps=0.1, sgs=1, cs2=0.0289, delta=delta, scalar=True,
+                grad=1.0, schmidt=sc)
+    v = orc.initialize(n, n, n, 1, 4.0, 1.0)
+    phi = orc.initialize(n, n, n, -10, 3.0, 1.0)[0]
+    got = s.rhs(*[a.copy() for a in v], phi.copy())
+    want = o.rhs(*[a.copy() for a in v], phi.copy())
+    k2 = orc.wavenumber(n, n, n)[3].astype(np.float64)
+    k2[0, 0, 0] = 0
+    lin = [nu * k2] * 3 + [nu / sc * k2]
+    for a, b, y, c in zip(got, want, (*v, phi), lin):
+        assert relerr(a, b - c * y) < 1e-11
+    s.close()
+    p.close()
+
+
 def _les_pair(n, dynamic, nsteps, iupdate=2):
     nu, dt = 1e-3, 0.002
     delta = 2 * np.pi / n

@@ -63,6 +63,7 @@ def test_derivs():
 
 
 def test_les_fp64():
+    les.test_les_scalar_rhs_fp64()
     les.test_smag_and_static_dynsmag_fp64()
     les.test_dynsmag_update_fp64()
     for dynamic in (False, True):
