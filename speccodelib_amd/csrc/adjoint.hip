@@ -2,6 +2,7 @@
 // recomputation into a stage tape, reverse sweep of vector-Jacobian products.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "adjoint.hpp"
@@ -37,6 +38,7 @@ struct scl_adjoint {
   void* fields = nullptr;     // forward: V(3) DU(3) S(3) W(nw)
   void* work = nullptr;       // adjoint: WA(L.nin()) LAM(3) VB(3) Y(3)
   void* tape = nullptr;       // ckpt steps x 4 stage states x 3 fields
+  void* ptape = nullptr;      // LES: ckpt steps x 4 stages x 12 physical fields (u, w, S_ij), or none
   void* ckpts = nullptr;      // nseg x 3 fields
   void* staging = nullptr;    // one field in the reference layout
   double* coef = nullptr;     // 8 doubles (cs2 at [5])
@@ -99,8 +101,18 @@ cpx<R>* tape_slot(const scl_adjoint* a, int step, int stage) {
   return fld<R>(a->tape, fs, 3 * (4 * step + stage));
 }
 
+// Physical-space inputs of the pointwise pass of stage `stage` (1..4) of a
+// step, or null without the physical tape.
+template <class R>
+cpx<R>* ptape_slot(const scl_adjoint* a, int step, int stage) {
+  if (!a->ptape) return nullptr;
+  const int64_t fs = a->plan->fstride();
+  return fld<R>(a->ptape, fs, 12 * (4 * step + stage - 1));
+}
+
 // One forward step of the solver configuration on V; with tape_step >= 0
-// the stage states v, s2, s3, s4 go to the tape.
+// the stage states v, s2, s3, s4 go to the tape, and with the physical tape
+// each stage's u, w, S_ij in physical space to it.
 template <class R>
 int32_t fwd_step(scl_adjoint* a, int tape_step) {
   DevFields d = dev(a);
@@ -114,6 +126,7 @@ int32_t fwd_step(scl_adjoint* a, int tape_step) {
     if (tape_step >= 0) {
       st = tape_slot<R>(a, tape_step, stage - 1);
       d.s_out = stage < 4 ? (void*)tape_slot<R>(a, tape_step, stage) : nullptr;
+      d.phys_out = ptape_slot<R>(a, tape_step, stage);
     }
     SCL_CHECK(solver_stage<R>(p, d, stage, a->p, st));
   }
@@ -131,9 +144,10 @@ AdjParams<R> adj_params(const scl_adjoint* a) {
 }
 
 // J(s)^T applied to the Y field (P nbar, before the Hermitian transpose),
-// then the bookkeeping of `mode`.
+// then the bookkeeping of `mode`.  ph: u, w, S_ij of s in physical space
+// (the physical tape), or null to transform them here.
 template <class R>
-int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode) {
+int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullptr) {
   scl_plan3d* p = a->plan;
   const Geom& g = p->g;
   const int64_t fs = p->fstride();
@@ -148,7 +162,13 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode) {
   // prep fused into the z-inverse (1024-point z-lines: separate passes).
   // The combine stays a separate pass: fused with the z-forward it ran at
   // one wave per SIMD and made the 128^3 backward slower (568 vs 536 ms).
-  if (g.nz < 1024) {
+  if (ph) {
+    // only ybar and taubar (W[12..20]) are transformed
+    const int q0 = a->L.y_in();
+    SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L, false));
+    SCL_CHECK(Ops<R>::colfft(*p, WA + q0 * fs, fs, nin - q0, 1, +1));
+    SCL_CHECK(Ops<R>::row_c2r(*p, WA + q0 * fs, fs, nin - q0));
+  } else if (g.nz < 1024) {
     SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L));
     SCL_CHECK(Ops<R>::colfft(*p, WA, fs, nin, 1, +1));
     SCL_CHECK(Ops<R>::row_c2r(*p, WA, fs, nin));
@@ -164,8 +184,10 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode) {
   const R c = (R)(-2.0 * a->cs2) * d2 * scale;
   const R cc = (R)-2 * d2 * scale;
   const unsigned nb = (unsigned)((g.nmain + 255) / 256);
-  hipLaunchKernelGGL((k_adj_phys<R>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, a->L, scale, c, cc,
-                     a->L.les ? a->part : nullptr);
+  if (a->L.les)
+    hipLaunchKernelGGL((k_adj_phys<R, 1>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, a->part, ph);
+  else
+    hipLaunchKernelGGL((k_adj_phys<R, 0>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, nullptr, ph);
   SCL_CHECK(check_launch("adj_phys"));
   if (a->L.les) {
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, p->stream, a->part, (int)nb, a->acc);
@@ -233,7 +255,8 @@ int32_t backward(scl_adjoint* a, const void* const* gT, void* const* g0, double*
     for (int n = n1 - 1; n >= n0; --n) {
       hipLaunchKernelGGL((k_adj_init<R>), ga, dim3(256), 0, p->stream, LAM, VB, Y, fs, g, ap);
       SCL_CHECK(check_launch("adj_init"));
-      for (int stage = 4; stage >= 1; --stage) SCL_CHECK(vjp<R>(a, tape_slot<R>(a, n - n0, stage - 1), stage));
+      for (int stage = 4; stage >= 1; --stage)
+        SCL_CHECK(vjp<R>(a, tape_slot<R>(a, n - n0, stage - 1), stage, ptape_slot<R>(a, n - n0, stage)));
       // vbar is the gradient with respect to u_n: the next (earlier) step's lambda
       SCL_HIP(hipMemcpyAsync(LAM, VB, 3 * fs * sizeof(cpx<R>), hipMemcpyDeviceToDevice, p->stream));
     }
@@ -284,7 +307,7 @@ int32_t by_prec_a(const scl_adjoint* a, F&& f) {
 extern "C" int32_t scl_adjoint_destroy(scl_adjoint* a) {
   if (!a) return SCL_OK;
   if (a->plan && a->plan->stream) (void)hipStreamSynchronize(a->plan->stream);
-  for (void* ptr : {a->fields, a->work, a->tape, a->ckpts, a->staging}) {
+  for (void* ptr : {a->fields, a->work, a->tape, a->ptape, a->ckpts, a->staging}) {
     if (ptr) (void)hipFree(ptr);
   }
   if (a->coef) (void)hipFree(a->coef);
@@ -331,6 +354,26 @@ extern "C" int32_t scl_adjoint_create(scl_plan3d* p, const scl_dns_params* prm, 
   if (!ok) {
     scl_adjoint_destroy(a);
     return SCL_ENOMEM;
+  }
+  // The physical tape (LES on fused z-passes) spares the backward the
+  // inverse transforms of the state's 12 fields in every vector-Jacobian
+  // product; it is kept when it takes at most half the free device memory
+  // (8.2 GB at 128^3 FP32 with 20-step segments).  SCL_ADJ_PHYS_TAPE=0
+  // turns it off; =1 requires it (creation fails without it).
+  const char* pt = std::getenv("SCL_ADJ_PHYS_TAPE");
+  const bool tape_off = pt && pt[0] == '0', tape_req = pt && pt[0] == '1';
+  if (a->L.les && g.nz < 1024 && !tape_off) {
+    const size_t bytes = fb * 48 * (size_t)a->ckpt;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes <= fr / 2 && hipMalloc(&a->ptape, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      a->ptape = nullptr;
+    }
+  }
+  if (tape_req && !a->ptape) {
+    scl_adjoint_destroy(a);
+    return set_error("SCL_ADJ_PHYS_TAPE=1: no physical tape (it needs sgs = 1, z-lines below 1024 points and "
+                     "memory)"), SCL_EINVAL;
   }
   if (hipMemset(a->fields, 0, fb * (9 + a->nw)) != hipSuccess || hipMemset(a->coef, 0, 8 * sizeof(double)) != hipSuccess ||
       hipEventCreate(&a->e0) != hipSuccess || hipEventCreate(&a->e1) != hipSuccess) {

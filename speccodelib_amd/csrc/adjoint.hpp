@@ -28,10 +28,10 @@ namespace scl {
 // pointwise pass: ubar(3) wbar(3) [Sbar_ij(6)].
 struct AdjLayout {
   int les;
-  __host__ __device__ int nin() const { return 9 + 12 * les; }
-  __host__ __device__ int nout() const { return 6 + 6 * les; }
-  __host__ __device__ int y_in() const { return 6 + 6 * les; }
-  __host__ __device__ int tau_in() const { return 9 + 6 * les; }
+  __host__ __device__ constexpr int nin() const { return 9 + 12 * les; }
+  __host__ __device__ constexpr int nout() const { return 6 + 6 * les; }
+  __host__ __device__ constexpr int y_in() const { return 6 + 6 * les; }
+  __host__ __device__ constexpr int tau_in() const { return 9 + 6 * les; }
 };
 
 __device__ inline bool sym_zero_mode(const Geom& g, int x, int y, int z, int ik2) {
@@ -102,16 +102,22 @@ __global__ __launch_bounds__(256) void k_adj_prep(const cpx<R>* __restrict__ s, 
 //   m = 1 on the diagonal, 2 off it;
 // and the cs2 derivative sum over points of sigmabar_p dsigma_p/dcs2 =
 // cc |S| sum_p sigmabar_p S_p (cc = -2 D^2 scale), per block into part[].
-template <class R>
-__global__ __launch_bounds__(256) void k_adj_phys(cpx<R>* __restrict__ W, int64_t fs, Geom g, AdjLayout L, R scale,
-                                                  R c, R cc, double* __restrict__ part) {
+// The state inputs u, w, S_ij come from P (the physical-space tape of the
+// recomputed forward, stride fs) when it is given, else from W.  The
+// layout is a template parameter so that the per-thread arrays stay in
+// registers.
+template <class R, int LES>
+__global__ __launch_bounds__(256) void k_adj_phys(cpx<R>* __restrict__ W, int64_t fs, Geom g, R scale, R c, R cc,
+                                                  double* __restrict__ part, const cpx<R>* __restrict__ P) {
+  constexpr AdjLayout L{LES};
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double acc = 0.0;
   if (i < g.nmain) {
     cpx<R> in[21];
+    const cpx<R>* const st = P ? P : W;
 #pragma unroll
     for (int q = 0; q < 21; ++q)
-      if (q < L.nin()) in[q] = W[i + q * fs];
+      if (q < L.nin()) in[q] = (q < L.y_in() ? st : W)[i + q * fs];
     cpx<R> out[12];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -279,11 +285,12 @@ __global__ __launch_bounds__(256) void k_adj_combine(const cpx<R>* __restrict__ 
 
 // ZINV of the VJP inputs: k_adj_prep fused with the z-pass of their inverse
 // transform.  Per column tile the stage state s is loaded and u, w, S_ij
-// are formed in registers, z-transformed and stored; then Y is loaded and
-// ybar (zeroed on the symmetrize planes, times the inverse weight) and
-// taubar_ij are formed and transformed the same way.  The y- and x-passes
-// follow as for any field.
-template <class R, int N>
+// are formed in registers, z-transformed and stored (STATE; without it
+// they come from the physical-space tape of the recomputed forward and s
+// is not read); then Y is loaded and ybar (zeroed on the symmetrize
+// planes, times the inverse weight) and taubar_ij are formed and
+// transformed the same way.  The y- and x-passes follow as for any field.
+template <class R, int N, bool STATE>
 __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_adj_zinv(const cpx<R>* __restrict__ s,
                                                                const cpx<R>* __restrict__ Y, cpx<R>* __restrict__ W,
                                                                int64_t fs, Geom g, AdjLayout L,
@@ -310,43 +317,45 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_adj_zinv(const cpx<R>* _
     for (int m = 0; m < E; ++m) wq[zs.at(Ln::out_pos(m, rt))] = d[m];
   };
   cpx<R> v[3][E], d[E];
+  if constexpr (STATE) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const uint32_t o = zo.at(rt + T * m);
+    for (int m = 0; m < E; ++m) {
+      const uint32_t o = zo.at(rt + T * m);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) v[q][m] = t.ok ? (s + q * fs)[o] : mk<R>(0, 0);
-  }
-  // w = i k x u
-#pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = mul_i(ky * v[2][m] - kz(m) * v[1][m]);
-  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(d, 3);
-#pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[0][m] - kx * v[2][m]);
-  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(d, 4);
-#pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = mul_i(kx * v[1][m] - ky * v[0][m]);
-  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(d, 5);
-  if (L.les) {
-    constexpr int P[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int i = P[q][0], j = P[q][1];
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const R k[3] = {kx, ky, kz(m)};
-        d[m] = i == j ? mul_i(k[i] * v[i][m]) : mul_i((R)0.5 * (k[i] * v[j][m] + k[j] * v[i][m]));
-      }
-      Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-      store(d, 6 + q);
+      for (int q = 0; q < 3; ++q) v[q][m] = t.ok ? (s + q * fs)[o] : mk<R>(0, 0);
     }
-  }
+    // w = i k x u
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    Ln::run(v[q], lds, rt, twl, 1, ColIdx<CW>{c});
-    store(v[q], q);
+    for (int m = 0; m < E; ++m) d[m] = mul_i(ky * v[2][m] - kz(m) * v[1][m]);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, 3);
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[0][m] - kx * v[2][m]);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, 4);
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(kx * v[1][m] - ky * v[0][m]);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, 5);
+    if (L.les) {
+      constexpr int P[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int i = P[q][0], j = P[q][1];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const R k[3] = {kx, ky, kz(m)};
+          d[m] = i == j ? mul_i(k[i] * v[i][m]) : mul_i((R)0.5 * (k[i] * v[j][m] + k[j] * v[i][m]));
+        }
+        Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+        store(d, 6 + q);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      Ln::run(v[q], lds, rt, twl, 1, ColIdx<CW>{c});
+      store(v[q], q);
+    }
   }
   // ybar = Z(Y) times the inverse weights; v's registers are free again
   const R wi = (ikx == 0 || ikx == g.mx) ? (R)1 : (R)0.5;

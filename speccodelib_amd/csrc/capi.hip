@@ -639,7 +639,7 @@ static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const s
   }
   tm->mark(SCL_K_ZFWD);
   SCL_CHECK(ExtOps<R>::rhs(*p, mode, L, d.W<R>(), fs, d.V<R>(), d.DU<R>(), d.S<R>(), st, d.coef, ep,
-                           (R)(delta * delta)));
+                           (R)(delta * delta), reinterpret_cast<cpx<R>*>(d.phys_out)));
   // the kx = 0 Hermitian copy: as in rk_stage
   if (mode >= 1 && mode < 4 && !d.s_out) {
     tm->mark(-1);
@@ -1106,9 +1106,7 @@ static int32_t tau_static(scl_plan3d* p, cpx<R>* W, int64_t fs, double cs2, doub
   SCL_CHECK(check_launch("ext_prep"));
   SCL_CHECK(Ops<R>::c2r3d(*p, W, fs, L.nin()));
   const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
-  hipLaunchKernelGGL((k_ext_phys<R>), ExtOps<R>::grid_main(g), dim3(256), 0, p->stream, W, fs, g, L, scale,
-                     (R)(delta * delta), coefdev + 5);
-  SCL_CHECK(check_launch("ext_phys"));
+  SCL_CHECK(ExtOps<R>::phys(*p, W, fs, L, scale, (R)(delta * delta), coefdev + 5, nullptr));
   return Ops<R>::r2c3d(*p, W + L.tau_out() * fs, fs, 6);
 }
 

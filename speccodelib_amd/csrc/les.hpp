@@ -17,12 +17,12 @@ namespace scl {
 // outputs u x w(3) [tau_ij(6)] [u_j phi(3)].
 struct ExtLayout {
   int les, scalar;
-  __host__ __device__ int nin() const { return 6 + 6 * les + scalar; }
-  __host__ __device__ int nout() const { return 3 + 6 * les + 3 * scalar; }
-  __host__ __device__ int s_in() const { return 6; }                 // S_ij inputs
-  __host__ __device__ int phi_in() const { return 6 + 6 * les; }     // phi input
-  __host__ __device__ int tau_out() const { return 3; }              // tau_ij outputs
-  __host__ __device__ int flux_out() const { return 3 + 6 * les; }   // u_j phi outputs
+  __host__ __device__ constexpr int nin() const { return 6 + 6 * les + scalar; }
+  __host__ __device__ constexpr int nout() const { return 3 + 6 * les + 3 * scalar; }
+  __host__ __device__ constexpr int s_in() const { return 6; }                 // S_ij inputs
+  __host__ __device__ constexpr int phi_in() const { return 6 + 6 * les; }     // phi input
+  __host__ __device__ constexpr int tau_out() const { return 3; }              // tau_ij outputs
+  __host__ __device__ constexpr int flux_out() const { return 3 + 6 * les; }   // u_j phi outputs
 };
 
 __device__ inline void ks_of(const Geom& g, int64_t i, int& x, int& y, int& z, int& kx, int& ky, int& kz) {
@@ -81,9 +81,13 @@ __device__ inline R abs_strain(const R (&s)[6]) {
 // -2 cs2 D^2 |S| S_ij (dynsmag.f90:206-225), u_j phi (convecphi_dns.f90:23,
 // 33, 42), all times scale = 1/(nx ny nz).  cs2 comes from device memory
 // (the dynamic update writes it there).
-template <class R>
-__global__ __launch_bounds__(256) void k_ext_phys(cpx<R>* __restrict__ W, int64_t fs, Geom g, ExtLayout L, R scale,
-                                                  R d2, const double* __restrict__ cs2p) {
+// With tape given, the velocity inputs u, w and S_ij are also stored there
+// (stride fs): the adjoint's record of the recomputed forward.  The layout
+// is a template parameter so that the per-thread arrays stay in registers.
+template <class R, int LES, int SC>
+__global__ __launch_bounds__(256) void k_ext_phys(cpx<R>* __restrict__ W, int64_t fs, Geom g, R scale, R d2,
+                                                  const double* __restrict__ cs2p, cpx<R>* __restrict__ tape) {
+  constexpr ExtLayout L{LES, SC};
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= g.nmain) return;
   const R c = L.les ? (R)(-2.0 * cs2p[0]) * d2 * scale : (R)0;
@@ -91,6 +95,11 @@ __global__ __launch_bounds__(256) void k_ext_phys(cpx<R>* __restrict__ W, int64_
 #pragma unroll
   for (int q = 0; q < 13; ++q)
     if (q < L.nin()) in[q] = W[i + q * fs];
+  if (tape) {
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      if (q < L.s_in() + 6 * L.les) tape[i + q * fs] = in[q];
+  }
   cpx<R> out[12];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {  // the two packed reals

@@ -29,8 +29,10 @@ struct Ops {
   static int32_t ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
                           const double* coef, const ExtParams<R>& ep, const ExtLayout& L);
   // the adjoint VJP's z-inverse fused with its prep (k_adj_zinv); all
-  // fields share the stride fs
-  static int32_t adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L);
+  // fields share the stride fs; state false: only ybar and taubar (the
+  // state's u, w, S_ij come from a physical-space tape)
+  static int32_t adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L,
+                          bool state = true);
   static int32_t force_coef(const scl_plan3d& pl, const C* st, int64_t fstride, double fkmax, double eps, double eta,
                             int type, double* coef);
   static int32_t herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp);

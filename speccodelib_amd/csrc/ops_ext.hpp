@@ -12,10 +12,26 @@ struct ExtOps {
   static dim3 grid_all(const Geom& g) { return dim3((unsigned)((g.ntot + 255) / 256)); }
   static dim3 grid_main(const Geom& g) { return dim3((unsigned)((g.nmain + 255) / 256)); }
 
+  // The physical pointwise pass (k_ext_phys) of the layout L.
+  static int32_t phys(const scl_plan3d& pl, C* W, int64_t fs, ExtLayout L, R scale, R d2, const double* cs2p,
+                      C* tape) {
+    const Geom& g = pl.g;
+    if (L.les && L.scalar)
+      hipLaunchKernelGGL((k_ext_phys<R, 1, 1>), grid_main(g), dim3(256), 0, pl.stream, W, fs, g, scale, d2, cs2p, tape);
+    else if (L.les)
+      hipLaunchKernelGGL((k_ext_phys<R, 1, 0>), grid_main(g), dim3(256), 0, pl.stream, W, fs, g, scale, d2, cs2p, tape);
+    else if (L.scalar)
+      hipLaunchKernelGGL((k_ext_phys<R, 0, 1>), grid_main(g), dim3(256), 0, pl.stream, W, fs, g, scale, d2, cs2p, tape);
+    else
+      return set_error("ext_phys: neither a Smagorinsky term nor a scalar"), SCL_EINVAL;
+    return check_launch("ext_phys");
+  }
+
   // N(st) of the extended RHS into W (MODE 0) or one RK4 stage (MODE 1..4).
-  // W holds max(nin, nout) fields.
+  // W holds max(nin, nout) fields.  phys_tape (optional): where the
+  // physical-space u, w, S_ij of this evaluation are kept (k_ext_phys).
   static int32_t rhs(const scl_plan3d& pl, int mode, ExtLayout L, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
-                     const double* coef, const ExtParams<R>& ep, R delta2) {
+                     const double* coef, const ExtParams<R>& ep, R delta2, C* phys_tape = nullptr) {
     const Geom& g = pl.g;
     // 1024-point z-lines keep the separate prep and combine: the fused
     // z-passes spill there
@@ -31,8 +47,7 @@ struct ExtOps {
       SCL_CHECK(Ops<R>::c2r3d(pl, W, fs, L.nin()));
     }
     const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
-    hipLaunchKernelGGL((k_ext_phys<R>), grid_main(g), dim3(256), 0, pl.stream, W, fs, g, L, scale, delta2, coef + 5);
-    SCL_CHECK(check_launch("ext_phys"));
+    SCL_CHECK(phys(pl, W, fs, L, scale, delta2, coef + 5, phys_tape));
     if (fused) {
       // the x- and y-passes of the forward transform; the z-pass fused with
       // the combine

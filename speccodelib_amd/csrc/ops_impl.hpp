@@ -234,10 +234,13 @@ int32_t Ops<R>::ext_zinv(const scl_plan3d& pl, const C* st, C* W, int64_t fs, co
     constexpr int N = decltype(NC)::value;
     using CC = ColCfg<R, N>;
     SCL_CHECK((check_cols<R, N>(g)));
-    if constexpr (N >= 1024) return set_error("fused LES z-pass: 1024-point lines take the unfused path"), SCL_EINVAL;
-    hipLaunchKernelGGL((k_ext_zinv<R, N>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, st, W, fs, g, L,
-                       twp<R>(pl), pl.L / N);
-    return check_launch("ext_zinv");
+    if constexpr (N >= 1024) {
+      return set_error("fused LES z-pass: 1024-point lines take the unfused path"), SCL_EINVAL;
+    } else {
+      hipLaunchKernelGGL((k_ext_zinv<R, N>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, st, W, fs, g, L,
+                         twp<R>(pl), pl.L / N);
+      return check_launch("ext_zinv");
+    }
   });
 }
 
@@ -249,33 +252,44 @@ int32_t Ops<R>::ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V,
     constexpr int N = decltype(NC)::value;
     using CC = ColCfg<R, N>;
     SCL_CHECK((check_cols<R, N>(g)));
-    if constexpr (N >= 1024) return set_error("fused LES z-pass: 1024-point lines take the unfused path"), SCL_EINVAL;
-    const dim3 grd(col_grid<CC::CW, 2>(g)), blk(CC::BS);
-    const cpx<R>* tw = twp<R>(pl);
-    const int tws = pl.L / N;
-    switch (mode) {
-      case 0: hipLaunchKernelGGL((k_ext_zfwd<R, N, 0>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
-      case 1: hipLaunchKernelGGL((k_ext_zfwd<R, N, 1>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
-      case 2: hipLaunchKernelGGL((k_ext_zfwd<R, N, 2>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
-      case 3: hipLaunchKernelGGL((k_ext_zfwd<R, N, 3>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
-      case 4: hipLaunchKernelGGL((k_ext_zfwd<R, N, 4>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
-      default: set_error("bad mode %d", mode); return SCL_EINVAL;
+    if constexpr (N >= 1024) {
+      return set_error("fused LES z-pass: 1024-point lines take the unfused path"), SCL_EINVAL;
+    } else {
+      const dim3 grd(col_grid<CC::CW, 2>(g)), blk(CC::BS);
+      const cpx<R>* tw = twp<R>(pl);
+      const int tws = pl.L / N;
+      switch (mode) {
+        case 0: hipLaunchKernelGGL((k_ext_zfwd<R, N, 0>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 1: hipLaunchKernelGGL((k_ext_zfwd<R, N, 1>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 2: hipLaunchKernelGGL((k_ext_zfwd<R, N, 2>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 3: hipLaunchKernelGGL((k_ext_zfwd<R, N, 3>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 4: hipLaunchKernelGGL((k_ext_zfwd<R, N, 4>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        default: set_error("bad mode %d", mode); return SCL_EINVAL;
+      }
+      return check_launch("ext_zfwd");
     }
-    return check_launch("ext_zfwd");
   });
 }
 
 template <class R>
-int32_t Ops<R>::adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L) {
+int32_t Ops<R>::adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L,
+                         bool state) {
   const Geom& g = pl.g;
   return with_line<R>(g.nz, [&](auto NC) -> int32_t {
     constexpr int N = decltype(NC)::value;
     using CC = ColCfg<R, N>;
     SCL_CHECK((check_cols<R, N>(g)));
-    if constexpr (N >= 1024) return set_error("fused adjoint z-pass: 1024-point lines take the unfused path"), SCL_EINVAL;
-    hipLaunchKernelGGL((k_adj_zinv<R, N>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, Y, W, fs, g, L,
-                       twp<R>(pl), pl.L / N);
-    return check_launch("adj_zinv");
+    if constexpr (N >= 1024) {
+      return set_error("fused adjoint z-pass: 1024-point lines take the unfused path"), SCL_EINVAL;
+    } else {
+      if (state)
+        hipLaunchKernelGGL((k_adj_zinv<R, N, true>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, Y, W,
+                           fs, g, L, twp<R>(pl), pl.L / N);
+      else
+        hipLaunchKernelGGL((k_adj_zinv<R, N, false>), dim3(col_grid<CC::CW, 2>(g)), dim3(CC::BS), 0, pl.stream, s, Y, W,
+                           fs, g, L, twp<R>(pl), pl.L / N);
+      return check_launch("adj_zinv");
+    }
   });
 }
 

@@ -17,6 +17,7 @@ struct DevFields {
   int nc = 3;
   double* coef = nullptr;
   void* s_out = nullptr;
+  void* phys_out = nullptr;  // multi-pass RHS: physical u, w, S_ij kept here
   template <class R>
   cpx<R>* f(int i) const {
     return reinterpret_cast<cpx<R>*>(base) + i * fs;

@@ -162,6 +162,30 @@ def test_fp32_gradient_vs_fp64():
     assert err < 1e-4 and abs(c32 - c64) < 1e-4 * abs(c64)
 
 
+@pytest.mark.parametrize("prec", [scl.DP, scl.SP])
+def test_physical_tape_matches_transformed_state(prec, monkeypatch):
+    """The backward with the physical-space tape of the recomputed forward
+    (u, w, S_ij kept by the forward's pointwise pass) against the backward
+    that transforms them from the stage states in every product."""
+    n, nsteps, ckpt, cs2 = 32, 6, 4, 0.0225
+    print(f"case adjoint physical tape prec={prec}", flush=True)
+    k2 = _k2int(n)
+    cd = scl._cdtype(prec)
+    res = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("SCL_ADJ_PHYS_TAPE", mode)
+        p, a, u0, tgt = _run_case(n, nsteps, ckpt, 1, cs2, prec)
+        _, gT = loss_grad(a.forward(u0, cs2), tgt, k2)
+        res[mode] = a.backward([x.astype(cd) for x in gT])
+        a.close()
+        p.close()
+    (g1, c1), (g0, c0) = res["1"], res["0"]
+    err = max(np.abs(x - y).max() for x, y in zip(g1, g0)) / max(np.abs(y).max() for y in g0)
+    tol = 1e-12 if prec == scl.DP else 2e-6
+    print("tape vs no tape: grad rel err", err, "dcs2", c1, c0, flush=True)
+    assert err < tol and abs(c1 - c0) <= tol * abs(c0)
+
+
 def test_torch_wrapper_matches_c_abi():
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
@@ -187,8 +211,9 @@ def test_torch_wrapper_matches_c_abi():
     g0, dcs2 = a.backward(gT)
     gt = U0.grad.cpu().numpy()
     err = max(np.abs(gt[q] - g0[q]).max() for q in range(3)) / max(np.abs(x).max() for x in g0)
-    print("torch loss", float(L), L0, "grad rel err", err, "dCs", float(Cs.grad), 2 * 0.15 * dcs2, flush=True)
-    assert abs(float(L) - L0) <= 1e-12 * L0
+    Lv = float(L.detach())
+    print("torch loss", Lv, L0, "grad rel err", err, "dCs", float(Cs.grad), 2 * 0.15 * dcs2, flush=True)
+    assert abs(Lv - L0) <= 1e-12 * L0
     assert err < 1e-12
     assert abs(float(Cs.grad) - 2 * 0.15 * dcs2) <= 1e-10 * abs(dcs2)
     a.close()
