@@ -47,6 +47,7 @@ struct scl_adjoint {
   int nparts = 0;
   double cs2 = 0.0;
   bool have_forward = false;
+  bool last_taped = false;    // the tape holds the last segment of the last forward
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double fwd_ms = 0.0, bwd_ms = 0.0;
 };
@@ -214,13 +215,18 @@ int32_t forward(scl_adjoint* a, double cs2, const void* const* u0, void* const* 
   DevFields d = dev(a);
   SCL_CHECK(set_cs2<R>(a, cs2));
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, u0[q], d.V<R>() + q * fs, devp));
+  // The last segment is taped on the way: the backward starts with it and
+  // need not recompute it.
+  a->last_taped = false;
+  const int nlast = (a->nseg - 1) * a->ckpt;
   SCL_HIP(hipEventRecord(a->e0, p->stream));
   for (int n = 0; n < a->nsteps; ++n) {
     if (n % a->ckpt == 0)
       SCL_HIP(hipMemcpyAsync(fld<R>(a->ckpts, fs, 3 * (n / a->ckpt)), d.V<R>(), 3 * fs * sizeof(cpx<R>),
                              hipMemcpyDeviceToDevice, p->stream));
-    SCL_CHECK(fwd_step<R>(a, -1));
+    SCL_CHECK(fwd_step<R>(a, n >= nlast ? n - nlast : -1));
   }
+  a->last_taped = true;
   SCL_HIP(hipEventRecord(a->e1, p->stream));
   for (int q = 0; q < 3; ++q) SCL_CHECK(store_field<R>(a, d.V<R>() + q * fs, uT[q], devp));
   SCL_HIP(hipStreamSynchronize(p->stream));
@@ -249,9 +255,12 @@ int32_t backward(scl_adjoint* a, const void* const* gT, void* const* g0, double*
   SCL_HIP(hipEventRecord(a->e0, p->stream));
   for (int seg = a->nseg - 1; seg >= 0; --seg) {
     const int n0 = seg * a->ckpt, n1 = std::min(a->nsteps, n0 + a->ckpt);
-    SCL_HIP(hipMemcpyAsync(d.V<R>(), fld<R>(a->ckpts, fs, 3 * seg), 3 * fs * sizeof(cpx<R>),
-                           hipMemcpyDeviceToDevice, p->stream));
-    for (int n = n0; n < n1; ++n) SCL_CHECK(fwd_step<R>(a, n - n0));
+    if (!(seg == a->nseg - 1 && a->last_taped)) {
+      a->last_taped = false;
+      SCL_HIP(hipMemcpyAsync(d.V<R>(), fld<R>(a->ckpts, fs, 3 * seg), 3 * fs * sizeof(cpx<R>),
+                             hipMemcpyDeviceToDevice, p->stream));
+      for (int n = n0; n < n1; ++n) SCL_CHECK(fwd_step<R>(a, n - n0));
+    }
     for (int n = n1 - 1; n >= n0; --n) {
       hipLaunchKernelGGL((k_adj_init<R>), ga, dim3(256), 0, p->stream, LAM, VB, Y, fs, g, ap);
       SCL_CHECK(check_launch("adj_init"));
@@ -282,6 +291,7 @@ int32_t vjp_host(scl_adjoint* a, double cs2, const void* const* s, const void* c
   cpx<R>* VB = fld<R>(a->work, fs, nin + 3);
   cpx<R>* Y = fld<R>(a->work, fs, nin + 6);
   cpx<R>* S = fld<R>(a->tape, fs, 0);
+  a->last_taped = false;  // S overwrites the first tape slot
   SCL_CHECK(set_cs2<R>(a, cs2));
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, s[q], S + q * fs, devp));
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, nb[q], Y + q * fs, devp));
