@@ -11,7 +11,7 @@
 
 namespace scl {
 
-// Fixed-order sum of n partials added into *acc (one workgroup).
+// Fixed-order sum of n partials into *acc (one workgroup).
 __global__ __launch_bounds__(256) void k_adj_sum(const double* __restrict__ part, int n, double* __restrict__ acc) {
   __shared__ double red[256];
   double s = 0.0;
@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void k_adj_sum(const double* __restrict__ part
     if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *acc += red[0];
+  if (threadIdx.x == 0) *acc = red[0];
 }
 
 }  // namespace scl
@@ -134,6 +134,13 @@ int32_t fwd_step(scl_adjoint* a, int tape_step) {
   return SCL_OK;
 }
 
+// The cs2 derivative: the sum of the per-block partials that k_adj_phys
+// adds up over the products since part[] was cleared.
+int32_t sum_parts(scl_adjoint* a) {
+  hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, a->plan->stream, a->part, a->nparts, a->acc);
+  return check_launch("adj_sum");
+}
+
 template <class R>
 AdjParams<R> adj_params(const scl_adjoint* a) {
   AdjParams<R> ap;
@@ -190,10 +197,6 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullpt
   else
     hipLaunchKernelGGL((k_adj_phys<R, 0>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, nullptr, ph);
   SCL_CHECK(check_launch("adj_phys"));
-  if (a->L.les) {
-    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, p->stream, a->part, (int)nb, a->acc);
-    SCL_CHECK(check_launch("adj_sum"));
-  }
   const AdjParams<R> ap = adj_params<R>(a);
   SCL_CHECK(Ops<R>::r2c3d(*p, WA, fs, a->L.nout()));
   const dim3 ga((unsigned)((g.ntot + 255) / 256));
@@ -249,7 +252,7 @@ int32_t backward(scl_adjoint* a, const void* const* gT, void* const* g0, double*
   cpx<R>* VB = fld<R>(a->work, fs, nin + 3);
   cpx<R>* Y = fld<R>(a->work, fs, nin + 6);
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, gT[q], LAM + q * fs, devp));
-  SCL_HIP(hipMemsetAsync(a->acc, 0, sizeof(double), p->stream));
+  SCL_HIP(hipMemsetAsync(a->part, 0, a->nparts * sizeof(double), p->stream));
   const AdjParams<R> ap = adj_params<R>(a);
   const dim3 ga((unsigned)((g.ntot + 255) / 256));
   SCL_HIP(hipEventRecord(a->e0, p->stream));
@@ -270,6 +273,7 @@ int32_t backward(scl_adjoint* a, const void* const* gT, void* const* g0, double*
       SCL_HIP(hipMemcpyAsync(LAM, VB, 3 * fs * sizeof(cpx<R>), hipMemcpyDeviceToDevice, p->stream));
     }
   }
+  SCL_CHECK(sum_parts(a));
   SCL_HIP(hipEventRecord(a->e1, p->stream));
   for (int q = 0; q < 3; ++q) SCL_CHECK(store_field<R>(a, LAM + q * fs, g0[q], devp));
   double c = 0.0;
@@ -296,8 +300,9 @@ int32_t vjp_host(scl_adjoint* a, double cs2, const void* const* s, const void* c
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, s[q], S + q * fs, devp));
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, nb[q], Y + q * fs, devp));
   SCL_CHECK(Ops<R>::projection(*p, Y, fs));
-  SCL_HIP(hipMemsetAsync(a->acc, 0, sizeof(double), p->stream));
+  SCL_HIP(hipMemsetAsync(a->part, 0, a->nparts * sizeof(double), p->stream));
   SCL_CHECK(vjp<R>(a, S, 0));
+  SCL_CHECK(sum_parts(a));
   for (int q = 0; q < 3; ++q) SCL_CHECK(store_field<R>(a, VB + q * fs, out[q], devp));
   double c = 0.0;
   SCL_HIP(hipMemcpyAsync(&c, a->acc, sizeof(double), hipMemcpyDeviceToHost, p->stream));

@@ -101,7 +101,8 @@ __global__ __launch_bounds__(256) void k_adj_prep(const cpx<R>* __restrict__ s, 
 //   Sbar_q = c (|S| sigmabar_q + 2 m_q S_q (sum_p sigmabar_p S_p)/|S|),
 //   m = 1 on the diagonal, 2 off it;
 // and the cs2 derivative sum over points of sigmabar_p dsigma_p/dcs2 =
-// cc |S| sum_p sigmabar_p S_p (cc = -2 D^2 scale), per block into part[].
+// cc |S| sum_p sigmabar_p S_p (cc = -2 D^2 scale), per block added into
+// part[] (each block owns its slot; one sum over the slots at the end).
 // The state inputs u, w, S_ij come from P (the physical-space tape of the
 // recomputed forward, stride fs) when it is given, else from W.  The
 // layout is a template parameter so that the per-thread arrays stay in
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void k_adj_phys(cpx<R>* __restrict__ W, int64_
       if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
       __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) part[blockIdx.x] += red[0];
   }
 }
 

@@ -15,14 +15,17 @@ namespace scl {
 
 // Field counts of the extended RHS: inputs u(3) w(3) [S_ij(6)] [phi(1)],
 // outputs u x w(3) [tau_ij(6)] [u_j phi(3)].
+// Inputs u(0..2) w(3..5) [S_ij(6..11)] [phi]; outputs N(0..2) [u_j phi]
+// [tau_ij]: the fused x-pass writes each output field after the row's input
+// in that field has been read (the flux over w, tau over S_ij).
 struct ExtLayout {
   int les, scalar;
   __host__ __device__ constexpr int nin() const { return 6 + 6 * les + scalar; }
   __host__ __device__ constexpr int nout() const { return 3 + 6 * les + 3 * scalar; }
   __host__ __device__ constexpr int s_in() const { return 6; }                 // S_ij inputs
   __host__ __device__ constexpr int phi_in() const { return 6 + 6 * les; }     // phi input
-  __host__ __device__ constexpr int tau_out() const { return 3; }              // tau_ij outputs
-  __host__ __device__ constexpr int flux_out() const { return 3 + 6 * les; }   // u_j phi outputs
+  __host__ __device__ constexpr int flux_out() const { return 3; }             // u_j phi outputs
+  __host__ __device__ constexpr int tau_out() const { return 3 + 3 * scalar; } // tau_ij outputs
 };
 
 __device__ inline void ks_of(const Geom& g, int64_t i, int& x, int& y, int& z, int& kx, int& ky, int& kz) {
@@ -132,6 +135,107 @@ __global__ __launch_bounds__(256) void k_ext_phys(cpx<R>* __restrict__ W, int64_
 #pragma unroll
   for (int q = 0; q < 12; ++q)
     if (q < L.nout()) W[i + q * fs] = out[q];
+}
+
+// XPASS of the multi-pass RHS: per x-row, the c2r of the inputs, the
+// products of k_ext_phys and the r2c of the outputs, in place (the layout
+// lets every output overwrite an input the row has already read).  Phase 1
+// transforms u, w [phi] and writes N [and u_j phi]; phase 2 transforms
+// S_ij and writes tau_ij.  TAPE: u, w, S_ij in physical space also to tape,
+// in the layout k_row_c2r leaves (the adjoint's record of the forward).
+// The grid is persistent, as for k_row_convec.
+template <class R, int M, int LES, int SC, bool TAPE>
+__global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_ext(cpx<R>* W, int64_t fs, Geom g, R scale, R d2,
+                                                              const double* __restrict__ cs2p,
+                                                              cpx<R>* __restrict__ tape, const cpx<R>* __restrict__ tw,
+                                                              int twsM, int twsN) {
+  using C = RowCfg<R, M>;
+  constexpr ExtLayout L{LES, SC};
+  constexpr int T = C::TPR;
+  __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
+  __shared__ cpx<R> twM[M], twN[M];
+  for (int i = threadIdx.x; i < M; i += C::BS) {
+    twM[i] = tw[i * twsM];
+    twN[i] = tw[i * twsN];
+  }
+  __syncthreads();
+  const R c = LES ? (R)(-2.0 * cs2p[0]) * d2 * scale : (R)0;
+  const int slot = threadIdx.x / T, rt = threadIdx.x % T;
+  cpx<R>* lrow = lds + slot * lds_pad(M);
+  using Li = Line<R, M, 8, +1>;
+  using Lf = Line<R, M, 8, -1>;
+  const int64_t ngroups = g.nrow / C::RPW;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row = grp * C::RPW + slot;
+    auto load = [&](int f, cpx<R>(&raw)[8], cpx<R>& xm) {
+      const cpx<R>* base = W + f * fs;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) raw[m] = base[row * M + rt + T * m];
+      xm = rt == 0 ? base[g.nmain + row] : mk<R>(0, 0);
+    };
+    auto c2r = [&](cpx<R>(&raw)[8], cpx<R> xm, cpx<R>(&a)[8], int q) {
+      row_c2r_prep<R, M>(a, raw, xm, rt, twN, 1);
+      Li::run(a, lrow, rt, twM, 1, LinIdx{});
+      if constexpr (TAPE) {
+        if (q < 0) return;
+        cpx<R>* tb = tape + q * fs + row * M;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) tb[Li::out_pos(m, rt)] = a[m];
+      }
+    };
+    auto r2c = [&](cpx<R>(&p)[8], int f) {
+      cpx<R>* base = W + f * fs;
+      Lf::run(p, lrow, rt, twM, 1, LinIdx{});
+      row_r2c_post<R, M>(p, base + row * M, base + g.nmain + row, rt, twN, 1);
+    };
+    // phase 1: u, w -> N, then phi -> u_j phi
+    cpx<R> raw[6][8], xm[6], a[7][8], p[8];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) load(f, raw[f], xm[f]);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) c2r(raw[f], xm[f], a[f], f);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        p[m] = mk<R>((a[j1][m].x * a[3 + j2][m].x - a[j2][m].x * a[3 + j1][m].x) * scale,
+                     (a[j1][m].y * a[3 + j2][m].y - a[j2][m].y * a[3 + j1][m].y) * scale);
+      r2c(p, j);
+    }
+    if constexpr (SC != 0) {
+      load(L.phi_in(), raw[0], xm[0]);
+      c2r(raw[0], xm[0], a[6], -1);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          p[m] = mk<R>(a[j][m].x * a[6][m].x * scale, a[j][m].y * a[6][m].y * scale);
+        r2c(p, L.flux_out() + j);
+      }
+    }
+    if constexpr (LES != 0) {
+      // phase 2: S_ij -> tau_ij = c |S| S_ij
+      cpx<R> sraw[6][8], sxm[6], s[6][8];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) load(L.s_in() + q, sraw[q], sxm[q]);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) c2r(sraw[q], sxm[q], s[q], 6 + q);
+      cpx<R> ca[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const R sx[6] = {s[0][m].x, s[1][m].x, s[2][m].x, s[3][m].x, s[4][m].x, s[5][m].x};
+        const R sy[6] = {s[0][m].y, s[1][m].y, s[2][m].y, s[3][m].y, s[4][m].y, s[5][m].y};
+        ca[m] = mk<R>(c * abs_strain<R>(sx), c * abs_strain<R>(sy));
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) p[m] = mk<R>(ca[m].x * s[q][m].x, ca[m].y * s[q][m].y);
+        r2c(p, L.tau_out() + q);
+      }
+    }
+  }
 }
 
 // Parameters of the extended combine kernel.

@@ -26,6 +26,10 @@ struct Ops {
   // the LES / scalar RHS's z-passes fused with its prep and combine
   // (k_ext_zinv, k_ext_zfwd); st and W share the field stride fs
   static int32_t ext_zinv(const scl_plan3d& pl, const C* st, C* W, int64_t fs, const ExtLayout& L);
+  // the x-pass of the LES / scalar RHS with its physical products
+  // (k_row_ext); tape (optional): u, w, S_ij in physical space
+  static int32_t row_ext(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L, R scale, R d2,
+                         const double* cs2p, C* tape);
   static int32_t ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
                           const double* coef, const ExtParams<R>& ep, const ExtLayout& L);
   // the adjoint VJP's z-inverse fused with its prep (k_adj_zinv); all

@@ -37,25 +37,21 @@ struct ExtOps {
     // z-passes spill there
     const bool fused = g.nz < 1024;
     if (fused) {
-      // prep fused into the z-inverse; then the y- and x-passes
+      // prep fused into the z-inverse
       SCL_CHECK(Ops<R>::ext_zinv(pl, st, W, fs, L));
-      SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 1, +1));
-      SCL_CHECK(Ops<R>::row_c2r(pl, W, fs, L.nin()));
     } else {
       hipLaunchKernelGGL((k_ext_prep<R>), grid_all(g), dim3(256), 0, pl.stream, st, W, fs, g, L);
       SCL_CHECK(check_launch("ext_prep"));
-      SCL_CHECK(Ops<R>::c2r3d(pl, W, fs, L.nin()));
+      SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 2, +1));
     }
+    // y-inverse; the x-pass with the physical products; y-forward
     const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
-    SCL_CHECK(phys(pl, W, fs, L, scale, delta2, coef + 5, phys_tape));
-    if (fused) {
-      // the x- and y-passes of the forward transform; the z-pass fused with
-      // the combine
-      SCL_CHECK(Ops<R>::row_r2c(pl, W, fs, L.nout()));
-      SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 1, -1));
-      return Ops<R>::ext_zfwd(pl, mode, W, fs, V, DU, S, st, coef, ep, L);
-    }
-    SCL_CHECK(Ops<R>::r2c3d(pl, W, fs, L.nout()));
+    SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 1, +1));
+    SCL_CHECK(Ops<R>::row_ext(pl, W, fs, L, scale, delta2, coef + 5, phys_tape));
+    SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 1, -1));
+    // the z-forward fused with the combine, or separate
+    if (fused) return Ops<R>::ext_zfwd(pl, mode, W, fs, V, DU, S, st, coef, ep, L);
+    SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 2, -1));
     switch (mode) {
       case 0: hipLaunchKernelGGL((k_ext_combine<R, 0>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
       case 1: hipLaunchKernelGGL((k_ext_combine<R, 1>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;

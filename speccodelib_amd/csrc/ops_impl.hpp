@@ -338,6 +338,39 @@ int32_t Ops<R>::row_convec(const scl_plan3d& pl, C* W, int64_t fstride) {
 }
 
 template <class R>
+int32_t Ops<R>::row_ext(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L, R scale, R d2,
+                        const double* cs2p, C* tape) {
+  const Geom& g = pl.g;
+  if (!L.les && !L.scalar) return set_error("row_ext: neither a Smagorinsky term nor a scalar"), SCL_EINVAL;
+  return with_half<R>(g.mx, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    using RC = RowCfg<R, M>;
+    // persistent grid, as for row_convec (the tag gives each variant its own
+    // cached occupancy)
+    auto go = [&](auto tag, auto kern) -> int32_t {
+      (void)tag;
+      static int per_cu = 0;
+      if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, RC::BS, 0) != hipSuccess ||
+                      per_cu < 1))
+        per_cu = 1;
+      const int64_t groups = g.nrow / RC::RPW;
+      const int64_t grid = std::min<int64_t>(groups, (int64_t)per_cu * pl.ncu);
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W, fs, g, scale, d2, cs2p, tape,
+                         twp<R>(pl), pl.L / M, pl.L / g.nx);
+      return check_launch("row_ext");
+    };
+    if (tape) {
+      if (L.les && L.scalar) return go(IC<0>{}, k_row_ext<R, M, 1, 1, true>);
+      if (L.les) return go(IC<1>{}, k_row_ext<R, M, 1, 0, true>);
+      return set_error("row_ext: the tape needs the Smagorinsky term"), SCL_EINVAL;
+    }
+    if (L.les && L.scalar) return go(IC<2>{}, k_row_ext<R, M, 1, 1, false>);
+    if (L.les) return go(IC<3>{}, k_row_ext<R, M, 1, 0, false>);
+    return go(IC<4>{}, k_row_ext<R, M, 0, 1, false>);
+  });
+}
+
+template <class R>
 int32_t Ops<R>::row_c2r(const scl_plan3d& pl, C* W, int64_t fstride, int nf) {
   const Geom& g = pl.g;
   return with_half<R>(g.mx, [&](auto MC) -> int32_t {
