@@ -170,35 +170,42 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullpt
   // prep fused into the z-inverse (1024-point z-lines: separate passes).
   // The combine stays a separate pass: fused with the z-forward it ran at
   // one wave per SIMD and made the 128^3 backward slower (568 vs 536 ms).
-  if (ph) {
-    // only ybar and taubar (W[12..20]) are transformed
-    const int q0 = a->L.y_in();
-    SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L, false));
-    SCL_CHECK(Ops<R>::colfft(*p, WA + q0 * fs, fs, nin - q0, 1, +1));
-    SCL_CHECK(Ops<R>::row_c2r(*p, WA + q0 * fs, fs, nin - q0));
-  } else if (g.nz < 1024) {
-    SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L));
-    SCL_CHECK(Ops<R>::colfft(*p, WA, fs, nin, 1, +1));
-    SCL_CHECK(Ops<R>::row_c2r(*p, WA, fs, nin));
-  } else {
-    hipLaunchKernelGGL((k_adj_prep<R>), dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, p->stream, s, Y, WA, fs,
-                       g, a->L);
-    SCL_CHECK(check_launch("adj_prep"));
-    SCL_CHECK(Ops<R>::c2r3d(*p, WA, fs, nin));
-  }
   const double delta = a->p.delta > 0 ? a->p.delta : 2.0 * M_PI / g.nx;
   const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
   const R d2 = (R)(delta * delta);
   const R c = (R)(-2.0 * a->cs2) * d2 * scale;
   const R cc = (R)-2 * d2 * scale;
-  const unsigned nb = (unsigned)((g.nmain + 255) / 256);
-  if (a->L.les)
-    hipLaunchKernelGGL((k_adj_phys<R, 1>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, a->part, ph);
-  else
-    hipLaunchKernelGGL((k_adj_phys<R, 0>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, nullptr, ph);
-  SCL_CHECK(check_launch("adj_phys"));
   const AdjParams<R> ap = adj_params<R>(a);
-  SCL_CHECK(Ops<R>::r2c3d(*p, WA, fs, a->L.nout()));
+  if (ph) {
+    // only ybar and taubar (W[12..20]) are transformed; the x-pass reads
+    // u, w, S_ij from the tape and does the physical products
+    const int q0 = a->L.y_in();
+    SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L, false));
+    SCL_CHECK(Ops<R>::colfft(*p, WA + q0 * fs, fs, nin - q0, 1, +1));
+    SCL_CHECK(Ops<R>::row_adj(*p, WA, fs, scale, c, cc, a->part, ph));
+    SCL_CHECK(Ops<R>::colfft(*p, WA, fs, a->L.nout(), 1, -1));
+    SCL_CHECK(Ops<R>::colfft(*p, WA, fs, a->L.nout(), 2, -1));
+  } else {
+    if (g.nz < 1024) {
+      SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L));
+      SCL_CHECK(Ops<R>::colfft(*p, WA, fs, nin, 1, +1));
+      SCL_CHECK(Ops<R>::row_c2r(*p, WA, fs, nin));
+    } else {
+      hipLaunchKernelGGL((k_adj_prep<R>), dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, p->stream, s, Y, WA,
+                         fs, g, a->L);
+      SCL_CHECK(check_launch("adj_prep"));
+      SCL_CHECK(Ops<R>::c2r3d(*p, WA, fs, nin));
+    }
+    const unsigned nb = (unsigned)((g.nmain + 255) / 256);
+    if (a->L.les)
+      hipLaunchKernelGGL((k_adj_phys<R, 1>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, a->part,
+                         nullptr);
+    else
+      hipLaunchKernelGGL((k_adj_phys<R, 0>), dim3(nb), dim3(256), 0, p->stream, WA, fs, g, scale, c, cc, nullptr,
+                         nullptr);
+    SCL_CHECK(check_launch("adj_phys"));
+    SCL_CHECK(Ops<R>::r2c3d(*p, WA, fs, a->L.nout()));
+  }
   const dim3 ga((unsigned)((g.ntot + 255) / 256));
   switch (mode) {
     case 0: hipLaunchKernelGGL((k_adj_combine<R, 0>), ga, dim3(256), 0, p->stream, WA, fs, LAM, VB, Y, g, a->L, ap); break;

@@ -174,6 +174,164 @@ __global__ __launch_bounds__(256) void k_adj_phys(cpx<R>* __restrict__ W, int64_
   }
 }
 
+// XPASS of a vector-Jacobian product of the Smagorinsky RHS with the
+// physical tape: per x-row, the c2r of ybar and taubar (W[12..20]), the
+// products of k_adj_phys with u, w, S_ij read from P (stride fs, in the
+// layout k_row_c2r leaves), the r2c of ubar, wbar, Sbar_ij into W[0..11],
+// and the cs2 derivative's partial of the workgroup added into
+// part[blockIdx.x].  S_ij is read from P twice, one component at a time
+// (|S|^2 and the contraction with sigmabar accumulate together; then the
+// outputs), rather than kept in registers; the second read is of rows the
+// workgroup has just read.  The grid is persistent, as for k_row_convec.
+template <class R, int M>
+__global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_adj(cpx<R>* W, int64_t fs, Geom g, R scale, R c, R cc,
+                                                              double* __restrict__ part,
+                                                              const cpx<R>* __restrict__ P,
+                                                              const cpx<R>* __restrict__ tw, int twsM, int twsN) {
+  using C = RowCfg<R, M>;
+  constexpr AdjLayout L{1};
+  constexpr int T = C::TPR;
+  __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
+  __shared__ cpx<R> twM[M], twN[M];
+  __shared__ double red[C::BS / 64];
+  for (int i = threadIdx.x; i < M; i += C::BS) {
+    twM[i] = tw[i * twsM];
+    twN[i] = tw[i * twsN];
+  }
+  __syncthreads();
+  const int slot = threadIdx.x / T, rt = threadIdx.x % T;
+  cpx<R>* lrow = lds + slot * lds_pad(M);
+  using Li = Line<R, M, 8, +1>;
+  using Lf = Line<R, M, 8, -1>;
+  double acc = 0.0;
+  const int64_t ngroups = g.nrow / C::RPW;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row = grp * C::RPW + slot;
+    auto load = [&](int f, cpx<R>(&raw)[8], cpx<R>& xm) {
+      const cpx<R>* base = W + f * fs;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) raw[m] = base[row * M + rt + T * m];
+      xm = rt == 0 ? base[g.nmain + row] : mk<R>(0, 0);
+    };
+    auto c2r = [&](cpx<R>(&raw)[8], cpx<R> xm, cpx<R>(&a)[8]) {
+      row_c2r_prep<R, M>(a, raw, xm, rt, twN, 1);
+      Li::run(a, lrow, rt, twM, 1, LinIdx{});
+    };
+    // dep: a value the read must wait for (it orders the reads of S_ij
+    // one component at a time; the reads are never merged or hoisted)
+    auto tape = [&](int q, cpx<R>(&a)[8], R dep) {
+      const cpx<R>* tb = P + q * fs + row * M;
+      asm volatile("" : "+v"(tb) : "v"(dep) : "memory");
+#pragma unroll
+      for (int m = 0; m < 8; ++m) a[m] = tb[Li::out_pos(m, rt)];
+    };
+    auto r2c = [&](cpx<R>(&p)[8], int f) {
+      cpx<R>* base = W + f * fs;
+      Lf::run(p, lrow, rt, twM, 1, LinIdx{});
+      row_r2c_post<R, M>(p, base + row * M, base + g.nmain + row, rt, twN, 1);
+    };
+    // qbar = scale * c2r(ybar); ubar = w x qbar; wbar = qbar x u
+    cpx<R> qv[3][8], v[3][8], p[8];
+    {
+      cpx<R> raw[3][8], xm[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) load(L.y_in() + j, raw[j], xm[j]);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        c2r(raw[j], xm[j], qv[j]);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) qv[j][m] = scale * qv[j][m];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tape(3 + j, v[j], (R)0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        p[m] = mk<R>(v[j1][m].x * qv[j2][m].x - v[j2][m].x * qv[j1][m].x,
+                     v[j1][m].y * qv[j2][m].y - v[j2][m].y * qv[j1][m].y);
+      r2c(p, j);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tape(j, v[j], (R)0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        p[m] = mk<R>(qv[j1][m].x * v[j2][m].x - qv[j2][m].x * v[j1][m].x,
+                     qv[j1][m].y * v[j2][m].y - qv[j2][m].y * v[j1][m].y);
+      r2c(p, 3 + j);
+    }
+    // Sbar_q = c (|S| sigmabar_q + 2 m_q S_q (sum_p sigmabar_p S_p)/|S|)
+    cpx<R> av[8], tv[8], sb[6][8];
+    {
+      cpx<R> xm[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) load(L.tau_in() + q, sb[q], xm[q]);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        cpx<R> raw[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) raw[m] = sb[q][m];
+        c2r(raw, xm[q], sb[q]);
+      }
+    }
+    // |S|^2 = 2 (S11^2 + S22^2 + S33^2 + 2 (S12^2 + S13^2 + S23^2)) and t
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      av[m] = mk<R>(0, 0);
+      tv[m] = mk<R>(0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      tape(6 + q, v[0], tv[7].y);
+      const R w2 = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        av[m] = mk<R>(av[m].x + w2 * v[0][m].x * v[0][m].x, av[m].y + w2 * v[0][m].y * v[0][m].y);
+        tv[m] = mk<R>(tv[m].x + sb[q][m].x * v[0][m].x, tv[m].y + sb[q][m].y * v[0][m].y);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      av[m] = mk<R>(sqrt((R)2 * av[m].x), sqrt((R)2 * av[m].y));
+      acc += (double)cc * (double)av[m].x * (double)tv[m].x;
+      acc += (double)cc * (double)av[m].y * (double)tv[m].y;
+      // 2 t / |S|: FP32 through the hardware reciprocal (about 1 ulp): the
+      // expansion of 16 correctly rounded divisions side by side does not
+      // fit in the register budget of two waves per SIMD
+      if constexpr (sizeof(R) == 4)
+        tv[m] = mk<R>(av[m].x > (R)0 ? (R)2 * tv[m].x * __builtin_amdgcn_rcpf(av[m].x) : (R)0,
+                      av[m].y > (R)0 ? (R)2 * tv[m].y * __builtin_amdgcn_rcpf(av[m].y) : (R)0);
+      else
+        tv[m] = mk<R>(av[m].x > (R)0 ? (R)2 * tv[m].x / av[m].x : (R)0, av[m].y > (R)0 ? (R)2 * tv[m].y / av[m].y : (R)0);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const R mq = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
+      tape(6 + q, v[0], p[7].y);
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        p[m] = mk<R>(c * (av[m].x * sb[q][m].x + mq * tv[m].x * v[0][m].x),
+                     c * (av[m].y * sb[q][m].y + mq * tv[m].y * v[0][m].y));
+      r2c(p, 6 + q);
+    }
+  }
+  // fixed-order reduction of the workgroup's partial
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < C::BS / 64; ++w) s += red[w];
+    part[blockIdx.x] += s;
+  }
+}
+
 // Parameters of the adjoint stage bookkeeping.
 template <class R>
 struct AdjParams {

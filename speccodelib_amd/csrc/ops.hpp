@@ -37,6 +37,8 @@ struct Ops {
   // state's u, w, S_ij come from a physical-space tape)
   static int32_t adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L,
                           bool state = true);
+  // the x-pass of the Smagorinsky VJP with the physical tape P (k_row_adj)
+  static int32_t row_adj(const scl_plan3d& pl, C* W, int64_t fs, R scale, R c, R cc, double* part, const C* P);
   static int32_t force_coef(const scl_plan3d& pl, const C* st, int64_t fstride, double fkmax, double eps, double eta,
                             int type, double* coef);
   static int32_t herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp);

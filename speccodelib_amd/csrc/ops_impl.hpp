@@ -371,6 +371,24 @@ int32_t Ops<R>::row_ext(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout&
 }
 
 template <class R>
+int32_t Ops<R>::row_adj(const scl_plan3d& pl, C* W, int64_t fs, R scale, R c, R cc, double* part, const C* P) {
+  const Geom& g = pl.g;
+  return with_half<R>(g.mx, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    using RC = RowCfg<R, M>;
+    static int per_cu = 0;  // persistent grid, as for row_convec
+    if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_row_adj<R, M>, RC::BS, 0) != hipSuccess ||
+                    per_cu < 1))
+      per_cu = 1;
+    const int64_t groups = g.nrow / RC::RPW;
+    const int64_t grid = std::min<int64_t>(groups, (int64_t)per_cu * pl.ncu);
+    hipLaunchKernelGGL((k_row_adj<R, M>), dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W, fs, g, scale, c, cc,
+                       part, P, twp<R>(pl), pl.L / M, pl.L / g.nx);
+    return check_launch("row_adj");
+  });
+}
+
+template <class R>
 int32_t Ops<R>::row_c2r(const scl_plan3d& pl, C* W, int64_t fstride, int nf) {
   const Geom& g = pl.g;
   return with_half<R>(g.mx, [&](auto MC) -> int32_t {
