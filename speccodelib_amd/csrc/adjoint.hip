@@ -167,7 +167,8 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullpt
   const int64_t nh = (int64_t)(g.nz / 2 - 1) * g.ny + g.ny / 2 - 1;
   hipLaunchKernelGGL((k_adj_hermT<R>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, p->stream, Y, fs, g);
   SCL_CHECK(check_launch("adj_hermT"));
-  // prep fused into the z-inverse (1024-point z-lines: separate passes).
+  // prep fused into the z-inverse (1024-point z-lines, or SCL_EXT_UNFUSED:
+  // separate passes).
   // The combine stays a separate pass: fused with the z-forward it ran at
   // one wave per SIMD and made the 128^3 backward slower (568 vs 536 ms).
   const double delta = a->p.delta > 0 ? a->p.delta : 2.0 * M_PI / g.nx;
@@ -186,7 +187,7 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullpt
     SCL_CHECK(Ops<R>::colfft(*p, WA, fs, a->L.nout(), 1, -1));
     SCL_CHECK(Ops<R>::colfft(*p, WA, fs, a->L.nout(), 2, -1));
   } else {
-    if (g.nz < 1024) {
+    if (p->zfused) {
       SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L));
       SCL_CHECK(Ops<R>::colfft(*p, WA, fs, nin, 1, +1));
       SCL_CHECK(Ops<R>::row_c2r(*p, WA, fs, nin));
@@ -384,7 +385,7 @@ extern "C" int32_t scl_adjoint_create(scl_plan3d* p, const scl_dns_params* prm, 
   // turns it off; =1 requires it (creation fails without it).
   const char* pt = std::getenv("SCL_ADJ_PHYS_TAPE");
   const bool tape_off = pt && pt[0] == '0', tape_req = pt && pt[0] == '1';
-  if (a->L.les && g.nz < 1024 && !tape_off) {
+  if (a->L.les && p->zfused && !tape_off) {
     const size_t bytes = fb * 48 * (size_t)a->ckpt;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes <= fr / 2 && hipMalloc(&a->ptape, bytes) != hipSuccess) {

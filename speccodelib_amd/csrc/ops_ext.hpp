@@ -35,7 +35,7 @@ struct ExtOps {
     const Geom& g = pl.g;
     // 1024-point z-lines keep the separate prep and combine: the fused
     // z-passes spill there
-    const bool fused = g.nz < 1024;
+    const bool fused = pl.zfused;
     if (fused) {
       // prep fused into the z-inverse
       SCL_CHECK(Ops<R>::ext_zinv(pl, st, W, fs, L));

@@ -19,6 +19,10 @@ struct scl_plan3d {
   scl::Geom g{};
   int device = 0;
   int ncu = 256;  // compute units of the device
+  // the LES / scalar RHS and the adjoint products with their prep and
+  // combine fused into the z-passes (off for 1024-point z-lines, where
+  // those kernels spill)
+  bool zfused = true;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   void* tw = nullptr;  // twiddle table exp(-2 pi i j/L), j < L, in the plan's precision
