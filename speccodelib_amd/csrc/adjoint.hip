@@ -169,8 +169,9 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullpt
   SCL_CHECK(check_launch("adj_hermT"));
   // prep fused into the z-inverse (1024-point z-lines, or SCL_EXT_UNFUSED:
   // separate passes).
-  // The combine stays a separate pass: fused with the z-forward it ran at
-  // one wave per SIMD and made the 128^3 backward slower (568 vs 536 ms).
+  // The combine stays a separate pass: fused with the z-forward of the 12
+  // outputs it made the 128^3 backward slower, at one wave per SIMD (568 vs
+  // 536 ms) and again at two with a few spilled registers (343 vs 337 ms).
   const double delta = a->p.delta > 0 ? a->p.delta : 2.0 * M_PI / g.nx;
   const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
   const R d2 = (R)(delta * delta);

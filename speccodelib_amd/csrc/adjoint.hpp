@@ -384,9 +384,43 @@ __global__ __launch_bounds__(256) void k_adj_init(const cpx<R>* __restrict__ lam
   for (int q = 0; q < 3; ++q) Y[i + q * fs] = n[q];
 }
 
+// The stage bookkeeping of MODE on the entry at offset o, given sbar there
+// (0: vbar = sbar only).
+template <class R, int MODE>
+__device__ inline void adj_update(const cpx<R> (&sb)[3], int64_t o, int64_t fs, const cpx<R>* __restrict__ lam,
+                                  cpx<R>* __restrict__ vb, cpx<R>* __restrict__ Y, R kx, R ky, R kz, R E, R E2,
+                                  bool keep, const AdjParams<R>& p) {
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) vb[o + q * fs] = sb[q];
+  } else if constexpr (MODE == 1) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) vb[o + q * fs] = vb[o + q * fs] + sb[q];
+  } else {
+    cpx<R> n[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const cpx<R> gq = keep ? E2 * lam[o + q * fs] : mk<R>(0, 0);
+      if constexpr (MODE == 4) {
+        vb[o + q * fs] = vb[o + q * fs] + E2 * sb[q];
+        n[q] = (p.dt / E) * ((R)(1.0 / 3.0) * gq + E2 * sb[q]);
+      } else if constexpr (MODE == 3) {
+        vb[o + q * fs] = vb[o + q * fs] + E * sb[q];
+        n[q] = (p.dt / E) * ((R)(1.0 / 3.0) * gq + ((R)0.5 * E) * sb[q]);
+      } else {
+        vb[o + q * fs] = vb[o + q * fs] + E * sb[q];
+        n[q] = p.dt * ((R)(1.0 / 6.0) * gq + ((R)0.5 * E) * sb[q]);
+      }
+    }
+    project(n, kx, ky, kz);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Y[o + q * fs] = n[q];
+  }
+}
+
 // Spectral end of J^T: sbar = w (ubar^ + i k x wbar^ + S^T Sbar^) with the
 // c2r-adjoint weight w (2 interior, 1 on kx = 0 and kx = nx/2), then the
-// stage bookkeeping of MODE (0: vbar = sbar only).
+// stage bookkeeping of MODE.
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void k_adj_combine(const cpx<R>* __restrict__ W, int64_t fs,
                                                      const cpx<R>* __restrict__ lam, cpx<R>* __restrict__ vb,
@@ -414,32 +448,7 @@ __global__ __launch_bounds__(256) void k_adj_combine(const cpx<R>* __restrict__ 
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) sb[q] = wt * sb[q];
-  if constexpr (MODE == 0) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) vb[i + q * fs] = sb[q];
-  } else if constexpr (MODE == 1) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) vb[i + q * fs] = vb[i + q * fs] + sb[q];
-  } else {
-    cpx<R> n[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const cpx<R> gq = keep ? E2 * lam[i + q * fs] : mk<R>(0, 0);
-      if constexpr (MODE == 4) {
-        vb[i + q * fs] = vb[i + q * fs] + E2 * sb[q];
-        n[q] = (p.dt / E) * ((R)(1.0 / 3.0) * gq + E2 * sb[q]);
-      } else if constexpr (MODE == 3) {
-        vb[i + q * fs] = vb[i + q * fs] + E * sb[q];
-        n[q] = (p.dt / E) * ((R)(1.0 / 3.0) * gq + ((R)0.5 * E) * sb[q]);
-      } else {
-        vb[i + q * fs] = vb[i + q * fs] + E * sb[q];
-        n[q] = p.dt * ((R)(1.0 / 6.0) * gq + ((R)0.5 * E) * sb[q]);
-      }
-    }
-    project(n, kx, ky, kz);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) Y[i + q * fs] = n[q];
-  }
+  adj_update<R, MODE>(sb, i, fs, lam, vb, Y, kx, ky, kz, E, E2, keep, p);
 }
 
 // ZINV of the VJP inputs: k_adj_prep fused with the z-pass of their inverse

@@ -186,6 +186,29 @@ def test_physical_tape_matches_transformed_state(prec, monkeypatch):
     assert err < tol and abs(c1 - c0) <= tol * abs(c0)
 
 
+@pytest.mark.parametrize("nsteps,ckpt", [(6, 6), (7, 3)])
+def test_backward_after_vjp_and_repeated(nsteps, ckpt):
+    """The forward leaves its last segment taped; a vjp call in between
+    overwrites the tape and a second backward follows the first: every
+    backward must give the gradient of a backward right after the forward."""
+    n, cs2 = 32, 0.0225
+    print(f"case adjoint backward order nsteps={nsteps} ckpt={ckpt}", flush=True)
+    k2 = _k2int(n)
+    p, a, u0, tgt = _run_case(n, nsteps, ckpt, 1, cs2, scl.DP)
+    _, gT = loss_grad(a.forward(u0, cs2), tgt, k2)
+    g_ref, c_ref = a.backward(gT)
+    g_again, c_again = a.backward(gT)
+    a.forward(u0, cs2)
+    a.vjp(u0, _random_field(n, 7), cs2)
+    g_vjp, c_vjp = a.backward(gT)
+    for g, c in ((g_again, c_again), (g_vjp, c_vjp)):
+        err = max(np.abs(x - y).max() for x, y in zip(g, g_ref)) / max(np.abs(y).max() for y in g_ref)
+        print("grad rel diff", err, "dcs2", c, c_ref, flush=True)
+        assert err < 1e-13 and abs(c - c_ref) <= 1e-13 * abs(c_ref)
+    a.close()
+    p.close()
+
+
 def test_torch_wrapper_matches_c_abi():
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
