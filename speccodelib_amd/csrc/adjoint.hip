@@ -38,7 +38,8 @@ struct scl_adjoint {
   void* fields = nullptr;     // forward: V(3) DU(3) S(3) W(nw)
   void* work = nullptr;       // adjoint: WA(L.nin()) LAM(3) VB(3) Y(3)
   void* tape = nullptr;       // ckpt steps x 4 stage states x 3 fields
-  void* ptape = nullptr;      // LES: ckpt steps x 4 stages x 12 physical fields (u, w, S_ij), or none
+  void* ptape = nullptr;      // ckpt steps x 4 stages x pf physical fields (u, w [, S_ij]), or none
+  int pf = 6;                 // fields per stage on the physical tape: 6, 12 with the Smagorinsky term
   void* ckpts = nullptr;      // nseg x 3 fields
   void* staging = nullptr;    // one field in the reference layout
   double* coef = nullptr;     // 8 doubles (cs2 at [5])
@@ -108,7 +109,7 @@ template <class R>
 cpx<R>* ptape_slot(const scl_adjoint* a, int step, int stage) {
   if (!a->ptape) return nullptr;
   const int64_t fs = a->plan->fstride();
-  return fld<R>(a->ptape, fs, 12 * (4 * step + stage - 1));
+  return fld<R>(a->ptape, fs, a->pf * (4 * step + stage - 1));
 }
 
 // One forward step of the solver configuration on V; with tape_step >= 0
@@ -179,12 +180,12 @@ int32_t vjp(scl_adjoint* a, const cpx<R>* s, int mode, const cpx<R>* ph = nullpt
   const R cc = (R)-2 * d2 * scale;
   const AdjParams<R> ap = adj_params<R>(a);
   if (ph) {
-    // only ybar and taubar (W[12..20]) are transformed; the x-pass reads
-    // u, w, S_ij from the tape and does the physical products
+    // only ybar [and taubar] (from W[y_in]) are transformed; the x-pass
+    // reads u, w [, S_ij] from the tape and does the physical products
     const int q0 = a->L.y_in();
     SCL_CHECK(Ops<R>::adj_zinv(*p, s, Y, WA, fs, a->L, false));
     SCL_CHECK(Ops<R>::colfft(*p, WA + q0 * fs, fs, nin - q0, 1, +1));
-    SCL_CHECK(Ops<R>::row_adj(*p, WA, fs, scale, c, cc, a->part, ph));
+    SCL_CHECK(Ops<R>::row_adj(*p, WA, fs, scale, c, cc, a->part, ph, a->L.les));
     SCL_CHECK(Ops<R>::colfft(*p, WA, fs, a->L.nout(), 1, -1));
     SCL_CHECK(Ops<R>::colfft(*p, WA, fs, a->L.nout(), 2, -1));
   } else {
@@ -379,15 +380,17 @@ extern "C" int32_t scl_adjoint_create(scl_plan3d* p, const scl_dns_params* prm, 
     scl_adjoint_destroy(a);
     return SCL_ENOMEM;
   }
-  // The physical tape (LES on fused z-passes) spares the backward the
-  // inverse transforms of the state's 12 fields in every vector-Jacobian
-  // product; it is kept when it takes at most half the free device memory
-  // (8.2 GB at 128^3 FP32 with 20-step segments).  SCL_ADJ_PHYS_TAPE=0
-  // turns it off; =1 requires it (creation fails without it).
+  // The physical tape (plans with the fused z-passes) spares the backward
+  // the inverse transforms of the state's fields (u, w [, S_ij]: 6 or 12)
+  // in every vector-Jacobian product; it is kept when it takes at most half
+  // the free device memory (8.2 GB for the LES at 128^3 FP32 with 20-step
+  // segments).  SCL_ADJ_PHYS_TAPE=0 turns it off; =1 requires it (creation
+  // fails without it).
+  a->pf = a->L.les ? 12 : 6;
   const char* pt = std::getenv("SCL_ADJ_PHYS_TAPE");
   const bool tape_off = pt && pt[0] == '0', tape_req = pt && pt[0] == '1';
-  if (a->L.les && p->zfused && !tape_off) {
-    const size_t bytes = fb * 48 * (size_t)a->ckpt;
+  if (p->zfused && !tape_off) {
+    const size_t bytes = fb * 4 * a->pf * (size_t)a->ckpt;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess && bytes <= fr / 2 && hipMalloc(&a->ptape, bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -396,8 +399,8 @@ extern "C" int32_t scl_adjoint_create(scl_plan3d* p, const scl_dns_params* prm, 
   }
   if (tape_req && !a->ptape) {
     scl_adjoint_destroy(a);
-    return set_error("SCL_ADJ_PHYS_TAPE=1: no physical tape (it needs sgs = 1, z-lines below 1024 points and "
-                     "memory)"), SCL_EINVAL;
+    return set_error("SCL_ADJ_PHYS_TAPE=1: no physical tape (it needs z-lines below 1024 points and memory)"),
+           SCL_EINVAL;
   }
   if (hipMemset(a->fields, 0, fb * (9 + a->nw)) != hipSuccess || hipMemset(a->coef, 0, 8 * sizeof(double)) != hipSuccess ||
       hipEventCreate(&a->e0) != hipSuccess || hipEventCreate(&a->e1) != hipSuccess) {

@@ -174,22 +174,22 @@ __global__ __launch_bounds__(256) void k_adj_phys(cpx<R>* __restrict__ W, int64_
   }
 }
 
-// XPASS of a vector-Jacobian product of the Smagorinsky RHS with the
-// physical tape: per x-row, the c2r of ybar and taubar (W[12..20]), the
-// products of k_adj_phys with u, w, S_ij read from P (stride fs, in the
-// layout k_row_c2r leaves), the r2c of ubar, wbar, Sbar_ij into W[0..11],
-// and the cs2 derivative's partial of the workgroup added into
-// part[blockIdx.x].  S_ij is read from P twice, one component at a time
+// XPASS of a vector-Jacobian product with the physical tape: per x-row,
+// the c2r of ybar and taubar (W[12..20]; ybar alone at W[6..8] without the
+// Smagorinsky term), the products of k_adj_phys with u, w, S_ij read from P
+// (stride fs, in the layout k_row_c2r leaves), the r2c of ubar, wbar,
+// Sbar_ij into W[0..11], and the cs2 derivative's partial of the workgroup
+// added into part[blockIdx.x] (LES only).  S_ij is read from P twice, one component at a time
 // (|S|^2 and the contraction with sigmabar accumulate together; then the
 // outputs), rather than kept in registers; the second read is of rows the
 // workgroup has just read.  The grid is persistent, as for k_row_convec.
-template <class R, int M>
+template <class R, int M, int LES>
 __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_adj(cpx<R>* W, int64_t fs, Geom g, R scale, R c, R cc,
                                                               double* __restrict__ part,
                                                               const cpx<R>* __restrict__ P,
                                                               const cpx<R>* __restrict__ tw, int twsM, int twsN) {
   using C = RowCfg<R, M>;
-  constexpr AdjLayout L{1};
+  constexpr AdjLayout L{LES};
   constexpr int T = C::TPR;
   __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
   __shared__ cpx<R> twM[M], twN[M];
@@ -265,61 +265,65 @@ __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_adj(cpx<R>* W, int64
                      qv[j1][m].y * v[j2][m].y - qv[j2][m].y * v[j1][m].y);
       r2c(p, 3 + j);
     }
-    // Sbar_q = c (|S| sigmabar_q + 2 m_q S_q (sum_p sigmabar_p S_p)/|S|)
-    cpx<R> av[8], tv[8], sb[6][8];
-    {
-      cpx<R> xm[6];
+    if constexpr (LES != 0) {
+      // Sbar_q = c (|S| sigmabar_q + 2 m_q S_q (sum_p sigmabar_p S_p)/|S|)
+      cpx<R> av[8], tv[8], sb[6][8];
+      {
+        cpx<R> xm[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) load(L.tau_in() + q, sb[q], xm[q]);
+        for (int q = 0; q < 6; ++q) load(L.tau_in() + q, sb[q], xm[q]);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        cpx<R> raw[8];
+        for (int q = 0; q < 6; ++q) {
+          cpx<R> raw[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) raw[m] = sb[q][m];
-        c2r(raw, xm[q], sb[q]);
+          for (int m = 0; m < 8; ++m) raw[m] = sb[q][m];
+          c2r(raw, xm[q], sb[q]);
+        }
       }
-    }
-    // |S|^2 = 2 (S11^2 + S22^2 + S33^2 + 2 (S12^2 + S13^2 + S23^2)) and t
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      av[m] = mk<R>(0, 0);
-      tv[m] = mk<R>(0, 0);
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      tape(6 + q, v[0], tv[7].y);
-      const R w2 = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
+      // |S|^2 = 2 (S11^2 + S22^2 + S33^2 + 2 (S12^2 + S13^2 + S23^2)) and t
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        av[m] = mk<R>(av[m].x + w2 * v[0][m].x * v[0][m].x, av[m].y + w2 * v[0][m].y * v[0][m].y);
-        tv[m] = mk<R>(tv[m].x + sb[q][m].x * v[0][m].x, tv[m].y + sb[q][m].y * v[0][m].y);
+        av[m] = mk<R>(0, 0);
+        tv[m] = mk<R>(0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        tape(6 + q, v[0], tv[7].y);
+        const R w2 = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          av[m] = mk<R>(av[m].x + w2 * v[0][m].x * v[0][m].x, av[m].y + w2 * v[0][m].y * v[0][m].y);
+          tv[m] = mk<R>(tv[m].x + sb[q][m].x * v[0][m].x, tv[m].y + sb[q][m].y * v[0][m].y);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        av[m] = mk<R>(sqrt((R)2 * av[m].x), sqrt((R)2 * av[m].y));
+        acc += (double)cc * (double)av[m].x * (double)tv[m].x;
+        acc += (double)cc * (double)av[m].y * (double)tv[m].y;
+        // 2 t / |S|: FP32 through the hardware reciprocal (about 1 ulp): the
+        // expansion of 16 correctly rounded divisions side by side does not
+        // fit in the register budget of two waves per SIMD
+        if constexpr (sizeof(R) == 4)
+          tv[m] = mk<R>(av[m].x > (R)0 ? (R)2 * tv[m].x * __builtin_amdgcn_rcpf(av[m].x) : (R)0,
+                        av[m].y > (R)0 ? (R)2 * tv[m].y * __builtin_amdgcn_rcpf(av[m].y) : (R)0);
+        else
+          tv[m] = mk<R>(av[m].x > (R)0 ? (R)2 * tv[m].x / av[m].x : (R)0,
+                        av[m].y > (R)0 ? (R)2 * tv[m].y / av[m].y : (R)0);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const R mq = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
+        tape(6 + q, v[0], p[7].y);
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          p[m] = mk<R>(c * (av[m].x * sb[q][m].x + mq * tv[m].x * v[0][m].x),
+                       c * (av[m].y * sb[q][m].y + mq * tv[m].y * v[0][m].y));
+        r2c(p, 6 + q);
       }
     }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      av[m] = mk<R>(sqrt((R)2 * av[m].x), sqrt((R)2 * av[m].y));
-      acc += (double)cc * (double)av[m].x * (double)tv[m].x;
-      acc += (double)cc * (double)av[m].y * (double)tv[m].y;
-      // 2 t / |S|: FP32 through the hardware reciprocal (about 1 ulp): the
-      // expansion of 16 correctly rounded divisions side by side does not
-      // fit in the register budget of two waves per SIMD
-      if constexpr (sizeof(R) == 4)
-        tv[m] = mk<R>(av[m].x > (R)0 ? (R)2 * tv[m].x * __builtin_amdgcn_rcpf(av[m].x) : (R)0,
-                      av[m].y > (R)0 ? (R)2 * tv[m].y * __builtin_amdgcn_rcpf(av[m].y) : (R)0);
-      else
-        tv[m] = mk<R>(av[m].x > (R)0 ? (R)2 * tv[m].x / av[m].x : (R)0, av[m].y > (R)0 ? (R)2 * tv[m].y / av[m].y : (R)0);
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const R mq = (q == 0 || q == 3 || q == 5) ? (R)1 : (R)2;
-      tape(6 + q, v[0], p[7].y);
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        p[m] = mk<R>(c * (av[m].x * sb[q][m].x + mq * tv[m].x * v[0][m].x),
-                     c * (av[m].y * sb[q][m].y + mq * tv[m].y * v[0][m].y));
-      r2c(p, 6 + q);
-    }
   }
+  if constexpr (LES == 0) return;
   // fixed-order reduction of the workgroup's partial
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);

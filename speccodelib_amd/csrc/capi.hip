@@ -563,7 +563,7 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   tm->mark(SCL_K_YINV);
   SCL_CHECK(Ops<R>::yinv_vort(*p, W, fs));
   tm->mark(SCL_K_XPASS);
-  SCL_CHECK(Ops<R>::row_convec(*p, W, fs));
+  SCL_CHECK(Ops<R>::row_convec(*p, W, fs, reinterpret_cast<cpx<R>*>(d.phys_out)));
   tm->mark(SCL_K_YFWD);
   SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
   tm->mark(fuse ? SCL_K_ZFWD_ZINV : SCL_K_ZFWD);

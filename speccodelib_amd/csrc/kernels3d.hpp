@@ -686,10 +686,13 @@ __device__ inline void row_r2c_post(const cpx<R> (&z)[8], cpx<R>* mainrow, cpx<R
 }
 
 // XPASS: per row, c2r of 6 fields, u x w * scale, r2c of 3, in place into
-// fields 0..2.  Fields at W + f*fstride.
-template <class R, int M>
+// fields 0..2.  Fields at W + f*fstride.  TAPE: u, w in physical space also
+// to tape (stride fstride, the layout k_row_c2r leaves: the adjoint's record
+// of the forward).
+template <class R, int M, bool TAPE = false>
 __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_convec(cpx<R>* W, int64_t fstride, Geom g, R scale,
-                                                                 const cpx<R>* __restrict__ tw, int twsM, int twsN) {
+                                                                 const cpx<R>* __restrict__ tw, int twsM, int twsN,
+                                                                 cpx<R>* __restrict__ tape) {
   using C = RowCfg<R, M>;
   constexpr int T = C::TPR;
   __shared__ cpx<R> lds[C::RPW * lds_pad(M)];
@@ -723,6 +726,11 @@ __global__ __launch_bounds__((RowCfg<R, M>::BS)) void k_row_convec(cpx<R>* W, in
     for (int f = 0; f < 6; ++f) {
       row_c2r_prep<R, M>(a[f], raw[f], xm[f], rt, twN, 1);
       L::run(a[f], lrow, rt, twM, 1, LinIdx{});
+      if constexpr (TAPE) {
+        cpx<R>* tb = tape + f * fstride + row * M;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) tb[L::out_pos(m, rt)] = a[f][m];
+      }
     }
     // u x w on the packed reals: real and imaginary parts are the even and
     // odd x points (convec_dns.f90:30-42)

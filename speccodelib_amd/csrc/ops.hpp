@@ -38,11 +38,13 @@ struct Ops {
   static int32_t adj_zinv(const scl_plan3d& pl, const C* s, const C* Y, C* W, int64_t fs, const AdjLayout& L,
                           bool state = true);
   // the x-pass of the Smagorinsky VJP with the physical tape P (k_row_adj)
-  static int32_t row_adj(const scl_plan3d& pl, C* W, int64_t fs, R scale, R c, R cc, double* part, const C* P);
+  static int32_t row_adj(const scl_plan3d& pl, C* W, int64_t fs, R scale, R c, R cc, double* part, const C* P,
+                         int les);
   static int32_t force_coef(const scl_plan3d& pl, const C* st, int64_t fstride, double fkmax, double eps, double eta,
                             int type, double* coef);
   static int32_t herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp);
-  static int32_t row_convec(const scl_plan3d& pl, C* W, int64_t fstride);
+  // tape (optional): u, w in physical space (k_row_convec TAPE)
+  static int32_t row_convec(const scl_plan3d& pl, C* W, int64_t fstride, C* tape = nullptr);
   static int32_t row_c2r(const scl_plan3d& pl, C* W, int64_t fstride, int nf);
   static int32_t row_r2c(const scl_plan3d& pl, C* W, int64_t fstride, int nf);
   static int32_t ref_to_split(const scl_plan3d& pl, const C* ref, C* f);

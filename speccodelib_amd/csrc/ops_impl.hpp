@@ -314,26 +314,31 @@ int32_t Ops<R>::herm_fix(const scl_plan3d& pl, const FieldPtrs<R>& fp) {
 }
 
 template <class R>
-int32_t Ops<R>::row_convec(const scl_plan3d& pl, C* W, int64_t fstride) {
+int32_t Ops<R>::row_convec(const scl_plan3d& pl, C* W, int64_t fstride, C* tape) {
   const Geom& g = pl.g;
   const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
   return with_half<R>(g.mx, [&](auto MC) -> int32_t {
     constexpr int M = decltype(MC)::value;
     using RC = RowCfg<R, M>;
-    // persistent grid: as many workgroups as can be resident at once
-    static int resident = 0;
-    if (!resident) {
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_row_convec<R, M>, RC::BS, 0) != hipSuccess ||
-          per_cu < 1)
-        per_cu = 1;
-      resident = per_cu;
-    }
-    const int64_t groups = g.nrow / RC::RPW;
-    const int64_t grid = std::min<int64_t>(groups, (int64_t)resident * pl.ncu);
-    hipLaunchKernelGGL((k_row_convec<R, M>), dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W,
-                       fstride, g, scale, twp<R>(pl), pl.L / M, pl.L / g.nx);
-    return check_launch("row_convec");
+    // persistent grid: as many workgroups as can be resident at once (the
+    // tag gives each variant its own cached occupancy)
+    auto go = [&](auto tag, auto kern) -> int32_t {
+      (void)tag;
+      static int resident = 0;
+      if (!resident) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, RC::BS, 0) != hipSuccess || per_cu < 1)
+          per_cu = 1;
+        resident = per_cu;
+      }
+      const int64_t groups = g.nrow / RC::RPW;
+      const int64_t grid = std::min<int64_t>(groups, (int64_t)resident * pl.ncu);
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W, fstride, g, scale, twp<R>(pl),
+                         pl.L / M, pl.L / g.nx, tape);
+      return check_launch("row_convec");
+    };
+    if (tape) return go(IC<1>{}, k_row_convec<R, M, true>);
+    return go(IC<0>{}, k_row_convec<R, M, false>);
   });
 }
 
@@ -371,20 +376,27 @@ int32_t Ops<R>::row_ext(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout&
 }
 
 template <class R>
-int32_t Ops<R>::row_adj(const scl_plan3d& pl, C* W, int64_t fs, R scale, R c, R cc, double* part, const C* P) {
+int32_t Ops<R>::row_adj(const scl_plan3d& pl, C* W, int64_t fs, R scale, R c, R cc, double* part, const C* P,
+                        int les) {
   const Geom& g = pl.g;
   return with_half<R>(g.mx, [&](auto MC) -> int32_t {
     constexpr int M = decltype(MC)::value;
     using RC = RowCfg<R, M>;
-    static int per_cu = 0;  // persistent grid, as for row_convec
-    if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_row_adj<R, M>, RC::BS, 0) != hipSuccess ||
-                    per_cu < 1))
-      per_cu = 1;
-    const int64_t groups = g.nrow / RC::RPW;
-    const int64_t grid = std::min<int64_t>(groups, (int64_t)per_cu * pl.ncu);
-    hipLaunchKernelGGL((k_row_adj<R, M>), dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W, fs, g, scale, c, cc,
-                       part, P, twp<R>(pl), pl.L / M, pl.L / g.nx);
-    return check_launch("row_adj");
+    // persistent grid, as for row_convec
+    auto go = [&](auto tag, auto kern) -> int32_t {
+      (void)tag;
+      static int per_cu = 0;
+      if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, RC::BS, 0) != hipSuccess ||
+                      per_cu < 1))
+        per_cu = 1;
+      const int64_t groups = g.nrow / RC::RPW;
+      const int64_t grid = std::min<int64_t>(groups, (int64_t)per_cu * pl.ncu);
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RC::BS), 0, pl.stream, W, fs, g, scale, c, cc, part, P,
+                         twp<R>(pl), pl.L / M, pl.L / g.nx);
+      return check_launch("row_adj");
+    };
+    if (les) return go(IC<1>{}, k_row_adj<R, M, 1>);
+    return go(IC<0>{}, k_row_adj<R, M, 0>);
   });
 }
 

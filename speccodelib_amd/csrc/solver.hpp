@@ -17,7 +17,7 @@ struct DevFields {
   int nc = 3;
   double* coef = nullptr;
   void* s_out = nullptr;
-  void* phys_out = nullptr;  // multi-pass RHS: physical u, w, S_ij kept here
+  void* phys_out = nullptr;  // the x-pass keeps u, w [, S_ij] in physical space here (adjoint tape)
   template <class R>
   cpx<R>* f(int i) const {
     return reinterpret_cast<cpx<R>*>(base) + i * fs;

@@ -162,19 +162,20 @@ def test_fp32_gradient_vs_fp64():
     assert err < 1e-4 and abs(c32 - c64) < 1e-4 * abs(c64)
 
 
+@pytest.mark.parametrize("sgs", [0, 1])
 @pytest.mark.parametrize("prec", [scl.DP, scl.SP])
-def test_physical_tape_matches_transformed_state(prec, monkeypatch):
+def test_physical_tape_matches_transformed_state(prec, sgs, monkeypatch):
     """The backward with the physical-space tape of the recomputed forward
-    (u, w, S_ij kept by the forward's pointwise pass) against the backward
-    that transforms them from the stage states in every product."""
+    (u, w [, S_ij] kept by the forward's x-pass) against the backward that
+    transforms them from the stage states in every product."""
     n, nsteps, ckpt, cs2 = 32, 6, 4, 0.0225
-    print(f"case adjoint physical tape prec={prec}", flush=True)
+    print(f"case adjoint physical tape prec={prec} sgs={sgs}", flush=True)
     k2 = _k2int(n)
     cd = scl._cdtype(prec)
     res = {}
     for mode in ("1", "0"):
         monkeypatch.setenv("SCL_ADJ_PHYS_TAPE", mode)
-        p, a, u0, tgt = _run_case(n, nsteps, ckpt, 1, cs2, prec)
+        p, a, u0, tgt = _run_case(n, nsteps, ckpt, sgs, cs2, prec)
         _, gT = loss_grad(a.forward(u0, cs2), tgt, k2)
         res[mode] = a.backward([x.astype(cd) for x in gT])
         a.close()
