@@ -332,9 +332,11 @@ __global__ __launch_bounds__(256) void k_ext_combine(cpx<R>* W, int64_t fs, cpx<
 
 // ZINV of the extended inputs: k_ext_prep fused with the z-pass of their
 // inverse transform.  Per column tile the stage state u (st, st+fs, st+2fs)
-// is loaded once; each input field (w = i k x u, S_ij, then u itself and
-// phi at st+3fs) is formed in registers, z-transformed and stored to W at
-// its ExtLayout index; the y- and x-passes follow as for any field.
+// is loaded once; with the Smagorinsky term the fields that need kz are
+// formed in registers, z-transformed and stored: i kz u_x, i kz u_y (to
+// W[3], W[4]) and i kz u_z (= S_33, to W[11]), then u itself and phi at
+// st+3fs; k_yinv_les forms w and the other S_ij in the y-pass.  Without it
+// (scalar only) w = i k x u is formed here.
 template <class R, int N>
 __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zinv(const cpx<R>* __restrict__ st,
                                                                cpx<R>* __restrict__ W, int64_t fs, Geom g,
@@ -367,33 +369,29 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zinv(const cpx<R>* _
     for (int m = 0; m < E; ++m) wq[zs.at(Ln::out_pos(m, rt))] = d[m];
   };
   cpx<R> d[E];
-  // w = i k x u (as k_ext_prep)
-#pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = mul_i(ky * v[2][m] - kz(m) * v[1][m]);
-  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(d, 3);
-#pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[0][m] - kx * v[2][m]);
-  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(d, 4);
-#pragma unroll
-  for (int m = 0; m < E; ++m) d[m] = mul_i(kx * v[1][m] - ky * v[0][m]);
-  Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-  store(d, 5);
   if (L.les) {
-    // S_ij = (i/2)(k_i u_j + k_j u_i), order 11 12 13 22 23 33 (strain_of)
-    constexpr int P[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    // i kz u_j to W[3], W[4], W[L.s_in() + 5]
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int i = P[q][0], j = P[q][1];
+    for (int j = 0; j < 3; ++j) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const R k[3] = {kx, ky, kz(m)};
-        d[m] = i == j ? mul_i(k[i] * v[i][m]) : mul_i((R)0.5 * (k[i] * v[j][m] + k[j] * v[i][m]));
-      }
+      for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[j][m]);
       Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
-      store(d, L.s_in() + q);
+      store(d, j < 2 ? 3 + j : L.s_in() + 5);
     }
+  } else {
+    // w = i k x u (as k_ext_prep)
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(ky * v[2][m] - kz(m) * v[1][m]);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, 3);
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(kz(m) * v[0][m] - kx * v[2][m]);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, 4);
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(kx * v[1][m] - ky * v[0][m]);
+    Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+    store(d, 5);
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -405,6 +403,78 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zinv(const cpx<R>* _
     for (int m = 0; m < E; ++m) d[m] = t.ok ? (st + 3 * fs)[zo.at(rt + T * m)] : mk<R>(0, 0);
     Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
     store(d, L.phi_in());
+  }
+}
+
+// YINV of the LES inputs after k_ext_zinv: the products with kx and ky are
+// formed in the y-pass, before the y-inverse.  PART 0 loads u_x, u_y and
+// writes w_z, S_11, S_12, S_22 and u_x, u_y; PART 1 loads u_z, i kz u_x,
+// i kz u_y and writes w_x, w_y, S_13, S_23 and u_z (S_33 and phi take a
+// plain column pass).  The parts touch disjoint fields, and each reads its
+// fields before writing them.  Same S_ij and w as k_ext_prep (strain_of),
+// with the kz terms transformed along z first.
+template <class R, int N, int PART>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_les(cpx<R>* __restrict__ W, int64_t fs, Geom g,
+                                                               const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  constexpr ExtLayout L{1, 0};
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 1>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  using Ln = Line<R, N, E, +1>;
+  auto load = [&](cpx<R>(&v)[E], int f) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = t.ok ? W[t.off(rt + T * m) + f * fs] : mk<R>(0, 0);
+  };
+  auto put = [&](cpx<R>(&v)[E], int f) {
+    Ln::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
+    if (!t.ok) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) W[t.off(Ln::out_pos(m, rt)) + f * fs] = v[m];
+  };
+  const R kx = (R)t.kx;
+  auto ky = [&](int m) { return (R)kwave(rt + T * m, g.ny); };
+  const int s0 = L.s_in();
+  cpx<R> d[E];
+  if constexpr (PART == 0) {
+    cpx<R> a[E], b[E];  // u_x, u_y
+    load(a, 0);
+    load(b, 1);
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(kx * b[m] - ky(m) * a[m]);
+    put(d, 5);  // w_z
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(kx * a[m]);
+    put(d, s0);  // S_11
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i((R)0.5 * (kx * b[m] + ky(m) * a[m]));
+    put(d, s0 + 1);  // S_12
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(ky(m) * b[m]);
+    put(d, s0 + 3);  // S_22
+    put(a, 0);
+    put(b, 1);
+  } else {
+    cpx<R> a[E], p[E], q[E];  // u_z, i kz u_x, i kz u_y
+    load(a, 2);
+    load(p, 3);
+    load(q, 4);
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = mul_i(ky(m) * a[m]) - q[m];
+    put(d, 3);  // w_x
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = p[m] - mul_i(kx * a[m]);
+    put(d, 4);  // w_y
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = (R)0.5 * (mul_i(kx * a[m]) + p[m]);
+    put(d, s0 + 2);  // S_13
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = (R)0.5 * (mul_i(ky(m) * a[m]) + q[m]);
+    put(d, s0 + 4);  // S_23
+    put(a, 2);
   }
 }
 

@@ -44,9 +44,11 @@ struct ExtOps {
       SCL_CHECK(check_launch("ext_prep"));
       SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 2, +1));
     }
-    // y-inverse; the x-pass with the physical products; y-forward
+    // y-inverse (fused: w and S_ij formed there); the x-pass with the
+    // physical products; y-forward
     const R scale = (R)1 / (R)((double)g.nx * g.ny * g.nz);
-    SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 1, +1));
+    if (fused) SCL_CHECK(Ops<R>::ext_yinv(pl, W, fs, L));
+    else SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 1, +1));
     SCL_CHECK(Ops<R>::row_ext(pl, W, fs, L, scale, delta2, coef + 5, phys_tape));
     SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 1, -1));
     // the z-forward fused with the combine, or separate

@@ -245,6 +245,24 @@ int32_t Ops<R>::ext_zinv(const scl_plan3d& pl, const C* st, C* W, int64_t fs, co
 }
 
 template <class R>
+int32_t Ops<R>::ext_yinv(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L) {
+  const Geom& g = pl.g;
+  if (!L.les) return colfft(pl, W, fs, L.nin(), 1, +1);
+  return with_line<R>(g.ny, [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    using CC = ColCfg<R, N>;
+    SCL_CHECK((check_cols<R, N>(g)));
+    const dim3 grd(col_grid<CC::CW, 1>(g)), blk(CC::BS);
+    hipLaunchKernelGGL((k_yinv_les<R, N, 0>), grd, blk, 0, pl.stream, W, fs, g, twp<R>(pl), pl.L / N);
+    SCL_CHECK(check_launch("yinv_les"));
+    hipLaunchKernelGGL((k_yinv_les<R, N, 1>), grd, blk, 0, pl.stream, W, fs, g, twp<R>(pl), pl.L / N);
+    SCL_CHECK(check_launch("yinv_les"));
+    // S_33 (= i kz u_z after the z-pass) and phi: a plain y-inverse
+    return colfft(pl, W + (L.s_in() + 5) * fs, fs, 1 + L.scalar, 1, +1);
+  });
+}
+
+template <class R>
 int32_t Ops<R>::ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
                          const double* coef, const ExtParams<R>& ep, const ExtLayout& L) {
   const Geom& g = pl.g;
