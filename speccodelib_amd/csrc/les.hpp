@@ -478,11 +478,91 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_les(cpx<R>* __restr
   }
 }
 
+// YFWD of the extended outputs (after the x-pass), folding the kx and ky
+// terms of the combine: per y-column tile each output is y-transformed in
+// turn; PART 0 (Smagorinsky term) accumulates N_i -= i (kx tau_i1 +
+// ky tau_i2) in registers and writes N' to W[0..2]; PART 1 (scalar)
+// accumulates R' = -sign i (kx F_x + ky F_y) and writes it to the first flux
+// slot.  The fields the z-pass still needs (tau_13, tau_23, tau_33, F_z) are
+// stored transformed in place.  The parts touch disjoint fields; each reads
+// a field of the tile before writing its slot.  PART 0 is held to two waves
+// per SIMD (a few registers spill at some line lengths).
+template <class R, int N, int PART, int SC>
+__global__ __launch_bounds__((ColCfg<R, N>::BS)) __attribute__((amdgpu_waves_per_eu(2))) void k_yfwd_ext(
+    cpx<R>* __restrict__ W, int64_t fs, Geom g, R sg, const cpx<R>* __restrict__ tw, int tws) {
+  using CC = ColCfg<R, N>;
+  constexpr int CW = CC::CW, E = CC::E, T = CC::T;
+  constexpr ExtLayout L{1, SC};
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];  // first read after a barrier inside the transform
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const Tile t = col_tile<CW, 1>(g, xcd_swizzle(blockIdx.x, gridDim.x), c);
+  using Ln = Line<R, N, E, -1>;
+  auto load = [&](cpx<R>(&v)[E], int f) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = t.ok ? W[t.off(rt + T * m) + f * fs] : mk<R>(0, 0);
+    Ln::run(v, lds, rt, twl, 1, ColIdx<CW>{c});
+  };
+  auto store = [&](cpx<R>(&v)[E], int f) {
+    if (!t.ok) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) W[t.off(Ln::out_pos(m, rt)) + f * fs] = v[m];
+  };
+  const R kx = (R)t.kx;
+  auto ky = [&](int m) { return (R)kwave(Ln::out_pos(m, rt), g.ny); };
+  cpx<R> f[E];
+  if constexpr (PART == 0) {
+    cpx<R> a[E], b[E], d[E];
+    load(a, 0);
+    load(b, 1);
+    load(d, 2);
+    const int t0 = L.tau_out();
+    load(f, t0);  // tau_11
+#pragma unroll
+    for (int m = 0; m < E; ++m) a[m] = a[m] - mul_i(kx * f[m]);
+    load(f, t0 + 1);  // tau_12
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      a[m] = a[m] - mul_i(ky(m) * f[m]);
+      b[m] = b[m] - mul_i(kx * f[m]);
+    }
+    load(f, t0 + 2);  // tau_13
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = d[m] - mul_i(kx * f[m]);
+    store(f, t0 + 2);
+    load(f, t0 + 3);  // tau_22
+#pragma unroll
+    for (int m = 0; m < E; ++m) b[m] = b[m] - mul_i(ky(m) * f[m]);
+    load(f, t0 + 4);  // tau_23
+#pragma unroll
+    for (int m = 0; m < E; ++m) d[m] = d[m] - mul_i(ky(m) * f[m]);
+    store(f, t0 + 4);
+    load(f, t0 + 5);  // tau_33
+    store(f, t0 + 5);
+    store(a, 0);
+    store(b, 1);
+    store(d, 2);
+  } else {
+    const int f0 = L.flux_out();
+    cpx<R> r[E];
+    load(f, f0);
+#pragma unroll
+    for (int m = 0; m < E; ++m) r[m] = sg * mul_i(kx * f[m]);
+    load(f, f0 + 1);
+#pragma unroll
+    for (int m = 0; m < E; ++m) r[m] = r[m] + sg * mul_i(ky(m) * f[m]);
+    load(f, f0 + 2);
+    store(f, f0 + 2);
+    store(r, f0);
+  }
+}
+
 // ZFWD of the extended outputs: the z-pass of their forward transform fused
-// with k_ext_combine.  Each output field (u x w, tau_ij, u_j phi; already
-// x- and y-transformed) is z-transformed in turn and folded into the three
-// velocity components N and the scalar rate R_phi held in registers
-// (N_i -= i k_j tau_ij, R_phi -= i k.F, term by term), then the combine's
+// with k_ext_combine.  After k_yfwd_ext (x- and y-transformed, the kx and ky
+// terms folded) each remaining field is z-transformed in turn and folded
+// into the three velocity components N and the scalar rate R_phi held in
+// registers (N_i -= i kz tau_i3, R_phi -= sign i kz F_z), then the combine's
 // epilogue runs per entry: forcing, symmetrize's zeroing, projection, the
 // scalar source -G u_z, the optional cut, and the IF-RK4 stage (MODE 1..4)
 // or N, R_phi into W[0..3] (MODE 0).  Every field's columns are read before
@@ -513,52 +593,31 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zfwd(cpx<R>* W, int6
     for (int m = 0; m < E; ++m) d[m] = t.ok ? wq[zl.at(rt + T * m)] : mk<R>(0, 0);
     Ln::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
   };
+  // k_yfwd_ext has folded the kx and ky terms: N' at 0..2, R' at the
+  // first flux slot; the kz terms remain (tau_13, tau_23, tau_33, F_z)
   cpx<R> a[E], b[E], dz[E], rp[E], f[E];
   load(a, 0);
   load(b, 1);
   load(dz, 2);
   if (L.les) {
-    // tau order 11 12 13 22 23 33: N_i -= i k_j tau_ij
+    // N_i -= i kz tau_i3
     const int t0 = L.tau_out();
-    load(f, t0);  // tau_11
-#pragma unroll
-    for (int m = 0; m < E; ++m) a[m] = a[m] - mul_i(kx * f[m]);
-    load(f, t0 + 1);  // tau_12
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      a[m] = a[m] - mul_i(ky * f[m]);
-      b[m] = b[m] - mul_i(kx * f[m]);
-    }
     load(f, t0 + 2);  // tau_13
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      a[m] = a[m] - mul_i(kz(m) * f[m]);
-      dz[m] = dz[m] - mul_i(kx * f[m]);
-    }
-    load(f, t0 + 3);  // tau_22
-#pragma unroll
-    for (int m = 0; m < E; ++m) b[m] = b[m] - mul_i(ky * f[m]);
+    for (int m = 0; m < E; ++m) a[m] = a[m] - mul_i(kz(m) * f[m]);
     load(f, t0 + 4);  // tau_23
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      b[m] = b[m] - mul_i(kz(m) * f[m]);
-      dz[m] = dz[m] - mul_i(ky * f[m]);
-    }
+    for (int m = 0; m < E; ++m) b[m] = b[m] - mul_i(kz(m) * f[m]);
     load(f, t0 + 5);  // tau_33
 #pragma unroll
     for (int m = 0; m < E; ++m) dz[m] = dz[m] - mul_i(kz(m) * f[m]);
   }
   if (L.scalar) {
     // R_phi = -sign i k.F (convecphi_dns.f90:26-47's form, physical sign
-    // for the solver)
+    // for the solver): the kz term
     const int f0 = L.flux_out();
     const R sg = (R)(-p.sign_phi);
-    load(f, f0);
-#pragma unroll
-    for (int m = 0; m < E; ++m) rp[m] = sg * mul_i(kx * f[m]);
-    load(f, f0 + 1);
-#pragma unroll
-    for (int m = 0; m < E; ++m) rp[m] = rp[m] + sg * mul_i(ky * f[m]);
+    load(rp, f0);
     load(f, f0 + 2);
 #pragma unroll
     for (int m = 0; m < E; ++m) rp[m] = rp[m] + sg * mul_i(kz(m) * f[m]);

@@ -28,6 +28,9 @@ struct Ops {
   static int32_t ext_zinv(const scl_plan3d& pl, const C* st, C* W, int64_t fs, const ExtLayout& L);
   // the y-inverse after ext_zinv (k_yinv_les with the Smagorinsky term)
   static int32_t ext_yinv(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L);
+  // the y-forward before ext_zfwd, folding the kx, ky terms (k_yfwd_ext);
+  // sg = -sign of the scalar flux divergence
+  static int32_t ext_yfwd(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L, R sg);
   // the x-pass of the LES / scalar RHS with its physical products
   // (k_row_ext); tape (optional): u, w, S_ij in physical space
   static int32_t row_ext(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L, R scale, R d2,

@@ -50,9 +50,13 @@ struct ExtOps {
     if (fused) SCL_CHECK(Ops<R>::ext_yinv(pl, W, fs, L));
     else SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nin(), 1, +1));
     SCL_CHECK(Ops<R>::row_ext(pl, W, fs, L, scale, delta2, coef + 5, phys_tape));
+    // fused: the y-forward folds the kx, ky terms and the z-forward the kz
+    // terms and the combine; else plain passes and the combine kernel
+    if (fused) {
+      SCL_CHECK(Ops<R>::ext_yfwd(pl, W, fs, L, (R)(-ep.sign_phi)));
+      return Ops<R>::ext_zfwd(pl, mode, W, fs, V, DU, S, st, coef, ep, L);
+    }
     SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 1, -1));
-    // the z-forward fused with the combine, or separate
-    if (fused) return Ops<R>::ext_zfwd(pl, mode, W, fs, V, DU, S, st, coef, ep, L);
     SCL_CHECK(Ops<R>::colfft(pl, W, fs, L.nout(), 2, -1));
     switch (mode) {
       case 0: hipLaunchKernelGGL((k_ext_combine<R, 0>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;

@@ -263,6 +263,33 @@ int32_t Ops<R>::ext_yinv(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout
 }
 
 template <class R>
+int32_t Ops<R>::ext_yfwd(const scl_plan3d& pl, C* W, int64_t fs, const ExtLayout& L, R sg) {
+  const Geom& g = pl.g;
+  return with_line<R>(g.ny, [&](auto NC) -> int32_t {
+    constexpr int N = decltype(NC)::value;
+    using CC = ColCfg<R, N>;
+    SCL_CHECK((check_cols<R, N>(g)));
+    const dim3 grd(col_grid<CC::CW, 1>(g)), blk(CC::BS);
+    const cpx<R>* tw = twp<R>(pl);
+    const int tws = pl.L / N;
+    if (!L.les && !L.scalar) return set_error("ext_yfwd: neither a Smagorinsky term nor a scalar"), SCL_EINVAL;
+    if (L.les) {
+      if (L.scalar)
+        hipLaunchKernelGGL((k_yfwd_ext<R, N, 0, 1>), grd, blk, 0, pl.stream, W, fs, g, sg, tw, tws);
+      else
+        hipLaunchKernelGGL((k_yfwd_ext<R, N, 0, 0>), grd, blk, 0, pl.stream, W, fs, g, sg, tw, tws);
+      SCL_CHECK(check_launch("yfwd_ext"));
+    } else {
+      SCL_CHECK(colfft(pl, W, fs, 3, 1, -1));  // N alone
+    }
+    if (!L.scalar) return SCL_OK;
+    // the flux slots are W[3..5] with or without the Smagorinsky term
+    hipLaunchKernelGGL((k_yfwd_ext<R, N, 1, 1>), grd, blk, 0, pl.stream, W, fs, g, sg, tw, tws);
+    return check_launch("yfwd_ext");
+  });
+}
+
+template <class R>
 int32_t Ops<R>::ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
                          const double* coef, const ExtParams<R>& ep, const ExtLayout& L) {
   const Geom& g = pl.g;
