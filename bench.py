@@ -374,7 +374,8 @@ def main():
     ap.add_argument("--no-ens2d", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--heli1024", type=int, default=0, help="also time this many steps of the 1024^3 helical DNS")
-    ap.add_argument("--les256", type=int, default=0, help="also time this many steps of the 256^3 LES + scalar")
+    ap.add_argument("--les256", type=int, default=50,
+                    help="also time this many steps of the 256^3 LES + scalar (0: skip)")
     ap.add_argument("--no-main", action="store_true", help="skip the 512^3 DNS (profiling runs of the others)")
     ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
     ap.add_argument("--copy-calib", type=int, default=0,
