@@ -399,7 +399,7 @@ extern "C" int32_t scl_adjoint_create(scl_plan3d* p, const scl_dns_params* prm, 
   }
   if (tape_req && !a->ptape) {
     scl_adjoint_destroy(a);
-    return set_error("SCL_ADJ_PHYS_TAPE=1: no physical tape (it needs z-lines below 1024 points and memory)"),
+    return set_error("SCL_ADJ_PHYS_TAPE=1: no physical tape (it needs z- and y-lines below 1024 points and memory)"),
            SCL_EINVAL;
   }
   if (hipMemset(a->fields, 0, fb * (9 + a->nw)) != hipSuccess || hipMemset(a->coef, 0, 8 * sizeof(double)) != hipSuccess ||

@@ -244,10 +244,12 @@ extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t
   auto* p = new scl_plan3d();
   p->prec = prec;
   p->g = make_geom(nx, ny, nz, zs);
-  // SCL_EXT_UNFUSED=1: the separate prep and combine passes of 1024-point
-  // z-lines at any size, for tests of that path against the oracle
+  // The LES / scalar and adjoint passes with their pointwise work fused into
+  // the z- and y-passes, up to 512-point z- and y-lines (at 1024 points those
+  // kernels spill).  SCL_EXT_UNFUSED=1: the separate passes at any size, for
+  // tests of that path against the oracle
   const char* uf = std::getenv("SCL_EXT_UNFUSED");
-  p->zfused = nz < 1024 && !(uf && uf[0] == '1');
+  p->zfused = nz < 1024 && ny < 1024 && !(uf && uf[0] == '1');
   (void)hipGetDevice(&p->device);
   {
     int ncu = 0;

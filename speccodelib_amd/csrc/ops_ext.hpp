@@ -33,8 +33,8 @@ struct ExtOps {
   static int32_t rhs(const scl_plan3d& pl, int mode, ExtLayout L, C* W, int64_t fs, C* V, C* DU, C* S, const C* st,
                      const double* coef, const ExtParams<R>& ep, R delta2, C* phys_tape = nullptr) {
     const Geom& g = pl.g;
-    // 1024-point z-lines keep the separate prep and combine: the fused
-    // z-passes spill there
+    // 1024-point z- or y-lines keep the separate prep and combine: the
+    // fused passes spill there
     const bool fused = pl.zfused;
     if (fused) {
       // prep fused into the z-inverse

@@ -20,8 +20,8 @@ struct scl_plan3d {
   int device = 0;
   int ncu = 256;  // compute units of the device
   // the LES / scalar RHS and the adjoint products with their prep and
-  // combine fused into the z-passes (off for 1024-point z-lines, where
-  // those kernels spill)
+  // combine fused into the z- and y-passes (off for 1024-point z- or
+  // y-lines, where those kernels spill)
   bool zfused = true;
   hipStream_t stream = nullptr;
   bool own_stream = false;
