@@ -35,6 +35,10 @@ class SclError(RuntimeError):
         self.status = status
 
 
+class NonFiniteError(SclError):
+    """SCL_ENONFINITE: the solver's state holds a NaN or Inf."""
+
+
 class ScDnsParams(ctypes.Structure):
     _fields_ = [
         ("nu", ctypes.c_double),
@@ -181,7 +185,8 @@ def lib():
 
 def _check(status):
     if status != 0:
-        raise SclError(status, lib().scl_last_error().decode())
+        cls = NonFiniteError if status == -5 else SclError
+        raise cls(status, lib().scl_last_error().decode())
 
 
 def _ptr(a):
@@ -398,9 +403,13 @@ class DNS:
         _check(lib().scl_dns_helicity_spectrum(self.h, _ptr(hk), len(hk)))
         return hk
 
-    def diagnostics(self) -> dict:
+    def diagnostics(self, allow_nonfinite=False) -> dict:
+        """FP64 totals of the state.  A non-finite state raises
+        NonFiniteError unless allow_nonfinite (then ``nonfinite`` is 1)."""
         d = ScDiag()
-        _check(lib().scl_dns_diagnostics(self.h, ctypes.byref(d)))
+        st = lib().scl_dns_diagnostics(self.h, ctypes.byref(d))
+        if not (allow_nonfinite and st == -5):
+            _check(st)
         return {k: getattr(d, k) for k, _ in ScDiag._fields_}
 
     def rhs(self, *y):
@@ -554,6 +563,8 @@ class Adjoint:
         return [_ptr(a) for a in arrs]
 
     def forward(self, u0, cs2):
+        # the checkpoints now belong to this call, not to a torch forward
+        self._fwd_token = None
         out = [self.plan.zeros() for _ in range(3)]
         _check(lib().scl_adjoint_forward(self.h, cs2, *self._in(u0), *map(_ptr, out), 0))
         return tuple(out)
@@ -565,12 +576,14 @@ class Adjoint:
         return tuple(out), d.value
 
     def vjp(self, s, nbar, cs2):
+        self._fwd_token = None
         out = [self.plan.zeros() for _ in range(3)]
         d = ctypes.c_double()
         _check(lib().scl_adjoint_vjp(self.h, cs2, *self._in(s), *self._in(nbar), *map(_ptr, out), ctypes.byref(d), 0))
         return tuple(out), d.value
 
     def forward_device(self, cs2, u0_ptrs, uT_ptrs):
+        self._fwd_token = None
         _check(lib().scl_adjoint_forward(self.h, cs2, *u0_ptrs, *uT_ptrs, 1))
 
     def backward_device(self, gT_ptrs, g0_ptrs):

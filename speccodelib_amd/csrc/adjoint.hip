@@ -46,7 +46,8 @@ struct scl_adjoint {
   double* part = nullptr;     // per-block partials of the cs2 derivative
   double* acc = nullptr;      // the cs2 derivative
   int nparts = 0;
-  double cs2 = 0.0;
+  double cs2 = 0.0;           // C_s^2 of the kernels' coefficient slot (coef[5])
+  double fwd_cs2 = 0.0;       // C_s^2 of the last forward: the backward's
   bool have_forward = false;
   bool last_taped = false;    // the tape holds the last segment of the last forward
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -227,6 +228,7 @@ int32_t forward(scl_adjoint* a, double cs2, const void* const* u0, void* const* 
   const int64_t fs = p->fstride();
   DevFields d = dev(a);
   SCL_CHECK(set_cs2<R>(a, cs2));
+  a->fwd_cs2 = cs2;
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, u0[q], d.V<R>() + q * fs, devp));
   // The last segment is taped on the way: the backward starts with it and
   // need not recompute it.
@@ -261,6 +263,9 @@ int32_t backward(scl_adjoint* a, const void* const* gT, void* const* g0, double*
   cpx<R>* LAM = fld<R>(a->work, fs, nin);
   cpx<R>* VB = fld<R>(a->work, fs, nin + 3);
   cpx<R>* Y = fld<R>(a->work, fs, nin + 6);
+  // the coefficient slot may hold another C_s^2 since (scl_adjoint_vjp sets
+  // its own): recomputation and products run at the last forward's
+  SCL_CHECK(set_cs2<R>(a, a->fwd_cs2));
   for (int q = 0; q < 3; ++q) SCL_CHECK(load_field<R>(a, gT[q], LAM + q * fs, devp));
   SCL_HIP(hipMemsetAsync(a->part, 0, a->nparts * sizeof(double), p->stream));
   const AdjParams<R> ap = adj_params<R>(a);

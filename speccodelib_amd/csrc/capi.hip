@@ -615,7 +615,8 @@ static bool chain_stages(const scl_plan3d*, const DevFields& d) {
 
 template <class R>
 static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm) {
-  const RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  rp.nonfinite = d.nonfinite;
   const bool chain = chain_stages<R>(p, d);
   for (int stage = 1; stage <= 4; ++stage) {
     const cpx<R>* st = stage == 1 ? d.V<R>() : d.S<R>();
@@ -635,6 +636,7 @@ static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const s
   ExtLayout L{prm.sgs ? 1 : 0, prm.scalar ? 1 : 0};
   ExtParams<R> ep;
   ep.r = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  ep.r.nonfinite = d.nonfinite;
   ep.nu_phi = (R)(prm.nu / (prm.schmidt > 0 ? prm.schmidt : 1.0));
   ep.grad = (R)prm.grad;
   ep.sign_phi = 1;
@@ -735,9 +737,31 @@ extern "C" int32_t scl_initialize3d(int32_t prec, void* vx, void* vy, void* vz, 
 
 // ---------------------------------------------------------------------------
 // Device-resident solver
+static unsigned* nonfinite_flag(const scl_dns* d) { return reinterpret_cast<unsigned*>(d->coef + 8); }
+
 static DevFields dev_fields(const scl_dns* d) {
   DevFields f{d->fields, d->plan->fstride(), d->nc, d->coef};
+  f.nonfinite = nonfinite_flag(d);
   return f;
+}
+
+// Clears the sticky non-finite flag (a new state was set).
+static int32_t clear_nonfinite(scl_dns* d) {
+  SCL_HIP(hipMemsetAsync(nonfinite_flag(d), 0, sizeof(unsigned), d->plan->stream));
+  return SCL_OK;
+}
+
+// SCL_ENONFINITE if a stage 4 since the last clear wrote a non-finite v
+// (after the work queued so far has finished).
+static int32_t read_nonfinite(scl_dns* d) {
+  unsigned h = 0;
+  SCL_HIP(hipMemcpyAsync(&h, nonfinite_flag(d), sizeof(h), hipMemcpyDeviceToHost, d->plan->stream));
+  SCL_HIP(hipStreamSynchronize(d->plan->stream));
+  if (h) {
+    set_error("non-finite value in the solution (stepped from a non-finite or diverging state)");
+    return SCL_ENONFINITE;
+  }
+  return SCL_OK;
 }
 
 static void free_dns(scl_dns* d) {
@@ -778,7 +802,7 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
     return SCL_ENOMEM;
   };
   if (hipMalloc(&d->fields, bytes) != hipSuccess) return fail(bytes);
-  if (hipMalloc((void**)&d->coef, 8 * sizeof(double)) != hipSuccess) return fail(64);
+  if (hipMalloc((void**)&d->coef, 16 * sizeof(double)) != hipSuccess) return fail(128);
   if (prm->sgs == 2) {
     if (hipMalloc(&d->dyn, 24 * fb) != hipSuccess) return fail(24 * fb);
     if (hipMalloc(&d->gt, (size_t)p->fstride() * (p->esize() / 2)) != hipSuccess) return fail(fb / 2);
@@ -787,6 +811,7 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
   }
   double c0[8] = {0, 0, 0, 0, 0, prm->cs2, 0, 0};
   if (hipMemsetAsync(d->fields, 0, bytes, p->stream) != hipSuccess ||
+      hipMemsetAsync(d->coef, 0, 16 * sizeof(double), p->stream) != hipSuccess ||
       hipMemcpyAsync(d->coef, c0, sizeof(c0), hipMemcpyHostToDevice, p->stream) != hipSuccess) {
     (void)hipGetLastError();
     free_dns(d);
@@ -838,6 +863,7 @@ static int32_t dns_io(scl_dns* d, int first, int n, const void* const* in, void*
 extern "C" int32_t scl_dns_set_velocity(scl_dns* d, const void* vx, const void* vy, const void* vz) {
   if (!d) return set_error("null solver"), SCL_EINVAL;
   const void* vi[3] = {vx, vy, vz};
+  SCL_CHECK(clear_nonfinite(d));
   return dns_io(d, 0, 3, vi, nullptr);
 }
 
@@ -936,7 +962,7 @@ extern "C" int32_t scl_dns_step(scl_dns* d, int32_t nsteps) {
 extern "C" int32_t scl_dns_sync(scl_dns* d) {
   if (!d) return set_error("null solver"), SCL_EINVAL;
   SCL_HIP(hipStreamSynchronize(d->plan->stream));
-  return SCL_OK;
+  return read_nonfinite(d);
 }
 
 extern "C" double scl_dns_time(const scl_dns* d) { return d ? d->time : 0.0; }
@@ -1091,7 +1117,15 @@ extern "C" int32_t scl_dns_diagnostics(scl_dns* d, scl_diag* out) {
     double c[8];
     SCL_HIP(hipMemcpy(c, d->coef, sizeof(c), hipMemcpyDeviceToHost));
     out->cs2 = d->p.sgs ? c[5] : 0.0;
-    return divmax_run<R>(p, v, &out->divmax);
+    SCL_CHECK(divmax_run<R>(p, v, &out->divmax));
+    // every field of *out is filled; the status reports a non-finite state
+    const int32_t nf = read_nonfinite(d);
+    if (nf == SCL_OK && out->nonfinite) {
+      set_error("non-finite value in the solution");
+      return SCL_ENONFINITE;
+    }
+    if (nf == SCL_ENONFINITE) out->nonfinite = 1;
+    return nf;
   });
 }
 

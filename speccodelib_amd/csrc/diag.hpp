@@ -100,7 +100,10 @@ __global__ __launch_bounds__(256) void k_absmax(const cpx<R>* __restrict__ f, in
   double m = 0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const cpx<R> a = f[i];
-    m = fmax(m, fmax(fabs((double)a.x), fabs((double)a.y)));
+    // NaN counts as +inf (fmax would drop it)
+    const double ax = fabs((double)a.x), ay = fabs((double)a.y);
+    constexpr double big = std::numeric_limits<double>::max(), inf = std::numeric_limits<double>::infinity();
+    m = fmax(m, fmax(ax <= big ? ax : inf, ay <= big ? ay : inf));
   }
   __shared__ double red[256];
   red[threadIdx.x] = m;

@@ -317,9 +317,11 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
 // and E^2 y = y + expm1(2x) y, each exact to the rounding of the result.
 // Returns the new stage state (MODE 1..3, stored to S only where store_s)
 // or the new v (MODE 4, always stored).
+// MODE 4 with nf set: a non-finite new v raises the sticky flag *nf (read
+// first, so that a field full of NaN does not make every lane an atomic).
 template <int MODE, class R, class I>
 __device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
-                                    bool store_s) {
+                                    bool store_s, unsigned* nf = nullptr) {
   cpx<R> out;
   if constexpr (MODE == 1) {
     const cpx<R> v = V[o];
@@ -340,6 +342,7 @@ __device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>
   } else {
     out = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
     V[o] = out;
+    if (nf && !(isfinite(out.x) && isfinite(out.y)) && *(volatile unsigned*)nf == 0u) atomicOr(nf, 1u);
     return out;
   }
   if (store_s) S[o] = out;
@@ -347,8 +350,9 @@ __device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>
 }
 
 template <int MODE, class R, class I>
-__device__ inline void stage_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut) {
-  stage_keep<MODE, R>(a, o, V, DU, S, em1, x, cut, true);
+__device__ inline void stage_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
+                                    unsigned* nf = nullptr) {
+  stage_keep<MODE, R>(a, o, V, DU, S, em1, x, cut, true, nf);
 }
 
 // Parameters of the fused ZFWD epilogue.
@@ -360,6 +364,7 @@ struct RhsParams {
   R kc2;        // 2/3 cut: zero where k2 >= kc2
   int forced;   // 0 none, 1 force(), 2 force_heli()
   int dealias;  // 1: zero k2 >= kc2 in the RHS as well
+  unsigned* nonfinite = nullptr;  // sticky flag set by stage 4 on a non-finite v (null: no check)
 };
 
 // ZFWD: z-forward of the three transformed products + forcing + the
@@ -491,10 +496,11 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R em1 = expm1_r(x);
       const bool cut = k2 >= p.kc2;
       const bool store_s = !FUSED || (p.forced && k2 < p.fk2);
-      nx_[m] = stage_keep<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut, store_s);
-      ny_[m] = stage_keep<MODE, R>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut, store_s);
+      nx_[m] = stage_keep<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut, store_s, p.nonfinite);
+      ny_[m] = stage_keep<MODE, R>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut, store_s,
+                                   p.nonfinite);
       nz_[m] = stage_keep<MODE, R>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1, x, cut,
-                                   store_s);
+                                   store_s, p.nonfinite);
     }
   }
   if constexpr (FUSED && MODE > 0) {

@@ -320,12 +320,12 @@ __global__ __launch_bounds__(256) void k_ext_combine(cpx<R>* W, int64_t fs, cpx<
     const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
     const R em1 = expm1_r(xv);
     const bool cut = k2 >= p.r.kc2;
-    stage_update<MODE, R>(p.r.dt * a, i, V, DU, S, em1, xv, cut);
-    stage_update<MODE, R>(p.r.dt * b, i + fs, V, DU, S, em1, xv, cut);
-    stage_update<MODE, R>(p.r.dt * d, i + 2 * fs, V, DU, S, em1, xv, cut);
+    stage_update<MODE, R>(p.r.dt * a, i, V, DU, S, em1, xv, cut, p.r.nonfinite);
+    stage_update<MODE, R>(p.r.dt * b, i + fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
+    stage_update<MODE, R>(p.r.dt * d, i + 2 * fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
     if (L.scalar) {
       const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
-      stage_update<MODE, R>(p.r.dt * rp, i + 3 * fs, V, DU, S, expm1_r(xp), xp, cut);
+      stage_update<MODE, R>(p.r.dt * rp, i + 3 * fs, V, DU, S, expm1_r(xp), xp, cut, p.r.nonfinite);
     }
   }
 }
@@ -682,12 +682,12 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zfwd(cpx<R>* W, int6
       const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
       const R em1 = expm1_r(xv);
       const bool cut = k2 >= p.r.kc2;
-      stage_update<MODE, R>(p.r.dt * ax, o, V, DU, S, em1, xv, cut);
-      stage_update<MODE, R>(p.r.dt * by, o + fs, V, DU, S, em1, xv, cut);
-      stage_update<MODE, R>(p.r.dt * cz, o + 2 * fs, V, DU, S, em1, xv, cut);
+      stage_update<MODE, R>(p.r.dt * ax, o, V, DU, S, em1, xv, cut, p.r.nonfinite);
+      stage_update<MODE, R>(p.r.dt * by, o + fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
+      stage_update<MODE, R>(p.r.dt * cz, o + 2 * fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
       if (L.scalar) {
         const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
-        stage_update<MODE, R>(p.r.dt * r, o + 3 * fs, V, DU, S, expm1_r(xp), xp, cut);
+        stage_update<MODE, R>(p.r.dt * r, o + 3 * fs, V, DU, S, expm1_r(xp), xp, cut, p.r.nonfinite);
       }
     }
   }

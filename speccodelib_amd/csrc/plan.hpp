@@ -55,7 +55,8 @@ struct scl_dns {
   void* dyn = nullptr;       // dynamic coefficient: 24 scratch fields
   void* gt = nullptr;        // test filter (real, split layout)
   double* part = nullptr;    // FP64 partial sums of the dynamic update
-  double* coef = nullptr;    // forcing and Smagorinsky coefficients (8 doubles)
+  double* coef = nullptr;    // forcing and Smagorinsky coefficients (8 doubles), then the
+                             // sticky non-finite flag (an unsigned at coef + 8)
   double time = 0.0;
   int64_t steps = 0;
   hipGraph_t graph = nullptr;

@@ -18,6 +18,7 @@ struct DevFields {
   double* coef = nullptr;
   void* s_out = nullptr;
   void* phys_out = nullptr;  // the x-pass keeps u, w [, S_ij] in physical space here (adjoint tape)
+  unsigned* nonfinite = nullptr;  // sticky flag: stage 4 found a non-finite v (null: not checked)
   template <class R>
   cpx<R>* f(int i) const {
     return reinterpret_cast<cpx<R>*>(base) + i * fs;

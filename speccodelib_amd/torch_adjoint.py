@@ -28,13 +28,25 @@ def _ptrs(t):
     return [t[q].data_ptr() for q in range(3)]
 
 
+def _check_fields(t, adj: Adjoint, what):
+    """A (3, nz, ny, nx//2+1) complex CUDA tensor of the plan's precision:
+    the library reads and writes the plan's element size at each address, so
+    a complex64 tensor handed to an FP64 plan would be overrun."""
+    want = torch.complex64 if adj.plan.prec == _scl.SP else torch.complex128
+    if t.dtype != want:
+        raise ValueError(f"{what} has dtype {t.dtype}; this plan's precision needs {want}")
+    if t.dim() != 4 or t.shape[0] != 3:
+        raise ValueError(f"{what} must have shape (3, nz, ny, nx//2+1)")
+    if tuple(t.shape[1:]) != adj.plan.shape:
+        raise ValueError(f"{what} fields have shape {tuple(t.shape[1:])}, the plan expects {adj.plan.shape}")
+    if not t.is_cuda:
+        raise ValueError(f"{what} must be a CUDA tensor")
+
+
 class RK4Map(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, cs2, adj: Adjoint):
-        if not u0.is_cuda or not u0.is_complex() or u0.dim() != 4 or u0.shape[0] != 3:
-            raise ValueError("u0 must be a complex CUDA tensor of shape (3, nz, ny, nx//2+1)")
-        if tuple(u0.shape[1:]) != adj.plan.shape:
-            raise ValueError(f"u0 fields have shape {tuple(u0.shape[1:])}, the plan expects {adj.plan.shape}")
+        _check_fields(u0, adj, "u0")
         u0c = u0.detach().contiguous()
         uT = torch.empty_like(u0c)
         c = float(cs2)
@@ -56,6 +68,7 @@ class RK4Map(torch.autograd.Function):
             torch.cuda.synchronize(u0c.device)
             adj.forward_device(ctx.cs2, _ptrs(u0c), _ptrs(tmp))
             adj._fwd_token = ctx.token
+        _check_fields(gT, adj, "the gradient of u(T)")
         gTc = gT.contiguous()
         g0 = torch.empty_like(u0c)
         torch.cuda.synchronize(u0c.device)

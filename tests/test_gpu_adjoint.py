@@ -202,7 +202,12 @@ def test_backward_after_vjp_and_repeated(nsteps, ckpt):
     a.forward(u0, cs2)
     a.vjp(u0, _random_field(n, 7), cs2)
     g_vjp, c_vjp = a.backward(gT)
-    for g, c in ((g_again, c_again), (g_vjp, c_vjp)):
+    # a vjp at another C_s^2 between the forward and the backward: the
+    # backward still runs at the forward's
+    a.forward(u0, cs2)
+    a.vjp(u0, _random_field(n, 8), 4.0 * cs2)
+    g_vjp2, c_vjp2 = a.backward(gT)
+    for g, c in ((g_again, c_again), (g_vjp, c_vjp), (g_vjp2, c_vjp2)):
         err = max(np.abs(x - y).max() for x, y in zip(g, g_ref)) / max(np.abs(y).max() for y in g_ref)
         print("grad rel diff", err, "dcs2", c, c_ref, flush=True)
         assert err < 1e-13 and abs(c - c_ref) <= 1e-13 * abs(c_ref)
@@ -229,6 +234,10 @@ def test_torch_wrapper_matches_c_abi():
     H[:, :, 0] = 0.5
     UT = rk4_map(U0, Cs * Cs, a)
     L = (H * ((UT - T).abs() ** 2 + K2 * UT.abs() ** 2)).sum()
+    # a host forward of other data at another C_s^2 and a vjp in between:
+    # the torch backward must still differentiate its own forward
+    a.forward(tgt, 0.04)
+    a.vjp(u0, _random_field(n, 9), 0.09)
     L.backward()
     uT = a.forward(u0, 0.15 * 0.15)
     L0, gT = loss_grad(uT, tgt, k2)
