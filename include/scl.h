@@ -139,6 +139,10 @@ int32_t scl_convecphi_dns(scl_plan3d *plan, const void *vx, const void *vy, cons
                           void *wphi, const void *kx, const void *ky, const void *kz, int32_t lx1, int32_t ly,
                           int32_t lz, int32_t nx, int32_t ny, int32_t nz);
 
+/* skewness(ux,kx,s,nx,ny,nz) (skewness.f90:1-48): s = <d^3>/<d^2>^(3/2),
+ * d = du_x/dx in physical space, summed in FP64.  kx must be wavenumber()'s. */
+int32_t scl_skewness(scl_plan3d *plan, const void *ux, const void *kx, double *s, int32_t nx, int32_t ny, int32_t nz);
+
 /* initialize(vx,vy,vz,kx,ky,kz,k2,iseed,lx1,ly,lz,akp,u0,kcut)
  * (initialize.f90:373-463 draw order, :328-332 complex projection), host
  * only, in the real type of prec.  kcut <= 0 selects 2*lx/3. */
@@ -185,6 +189,7 @@ typedef struct scl_diag {
   double scalar_dissipation; /* 2 (nu/Sc) sum' k^2 |phi|^2 */
   double cs2;                /* current Smagorinsky coefficient (sgs > 0) */
   double helicity_dissipation; /* 2 nu sum' k^2 2 Re(w . conj v) */
+  double skewness;             /* <(du/dx)^3> / <(du/dx)^2>^(3/2) (skewness.f90:1-48) */
 } scl_diag;
 
 int32_t scl_dns_create(scl_plan3d *plan, const scl_dns_params *p, scl_dns **out);
@@ -199,7 +204,9 @@ int32_t scl_dns_initialize(scl_dns *d, int32_t iseed, double akp, double u0);
 int32_t scl_dns_step(scl_dns *d, int32_t nsteps);
 int32_t scl_dns_sync(scl_dns *d);
 double scl_dns_time(const scl_dns *d);
-/* E(ll), ll = 1..nbins (nbins <= nx/2), FP64 (spec.f90:66-89) */
+/* E(ll), ll = 1..nbins (nbins <= nx/2), FP64 (spec.f90:66-89).  The
+ * diagnostics sum in an order fixed by the grid: repeated calls on one state
+ * give the same bits. */
 int32_t scl_dns_spectrum(scl_dns *d, double *ek, int32_t nbins);
 /* H(ll) (helispec.f90:67-91) */
 int32_t scl_dns_helicity_spectrum(scl_dns *d, double *hk, int32_t nbins);

@@ -461,6 +461,14 @@ def helicity_spectrum(vx, vy, vz, kx, ky, kz, k2) -> np.ndarray:
     return np.bincount(ll[m] - 1, weights=h[m], minlength=lx)[:lx]
 
 
+def skewness(ux, kx, nx) -> float:
+    """Velocity-derivative skewness <d^3> / <d^2>^(3/2), d = du_x/dx in
+    physical space (skewness.f90:16-45), FP64 moments.  Pinned by
+    tests/test_oracle_pins.py."""
+    d = c2r(1j * kx[None, None, :].astype(np.float64) * ux.astype(np.complex128), nx)
+    return float(np.mean(d**3) / np.mean(d**2) ** 1.5)
+
+
 def dissipation(vx, vy, vz, k2, nu) -> float:
     """eps = 2 nu sum' k^2 |v|^2 = <2 nu s_ij s_ij> (viscenerdiss.f90:16-47 by
     Parseval, for solenoidal v)."""

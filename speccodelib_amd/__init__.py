@@ -74,6 +74,7 @@ class ScDiag(ctypes.Structure):
         ("scalar_dissipation", ctypes.c_double),
         ("cs2", ctypes.c_double),
         ("helicity_dissipation", ctypes.c_double),
+        ("skewness", ctypes.c_double),
     ]
 
 
@@ -147,6 +148,7 @@ def lib():
             "scl_smag": ([P] + [P] * 12 + [I32] * 6 + [D], I32),
             "scl_dynsmag": ([P] + [P] * 13 + [I32] * 7 + [D, D, P, P, P, I32], I32),
             "scl_convecphi_dns": ([P] + [P] * 8 + [I32] * 6, I32),
+            "scl_skewness": ([P, P, P, P, I32, I32, I32], I32),
             "scl_ens2d_create": ([I32, I32, P, P, ctypes.POINTER(ScEns2DParams), ctypes.POINTER(P)], I32),
             "scl_ens2d_destroy": ([P], I32),
             "scl_ens2d_set_member": ([P, I32, I32, P], I32),
@@ -310,6 +312,13 @@ class Plan3D:
         _check(lib().scl_convecphi_dns(self.h, *map(_ptr, (vx, vy, vz, phi, w, kx, ky, kz)), *self._dims(),
                                        self.nx, self.ny, self.nz))
         return w
+
+    def skewness(self, ux, kx):
+        """skewness.f90:1-48: <(du_x/dx)^3> / <(du_x/dx)^2>^(3/2), FP64."""
+        _field(ux, self.prec, self.shape)
+        s = ctypes.c_double()
+        _check(lib().scl_skewness(self.h, _ptr(ux), _ptr(kx), ctypes.byref(s), self.nx, self.ny, self.nz))
+        return s.value
 
     def wavenumber(self):
         return wavenumber(self.nx, self.ny, self.nz, self.prec)

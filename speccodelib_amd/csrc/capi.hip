@@ -1005,21 +1005,42 @@ extern "C" int32_t scl_dns_step_timed(scl_dns* d, int32_t nsteps, double* kind_m
 // Diagnostics
 template <class R>
 static int32_t diag_run(scl_plan3d* p, const cpx<R>* v, int nbins, std::vector<double>& res) {
-  const Geom& g = p->g;
   const int len = 2 * nbins + DIAG_TOT;
   void* buf;
-  SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * len * sizeof(double) + 64, &buf));
+  SCL_CHECK(plan_diag(p, (size_t)(diag_part_len(p->g.mx) + 2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &buf));
   double* part = (double*)buf;
-  double* fin = part + (size_t)DIAG_BLOCKS * len;
-  const size_t shbytes = (size_t)(4 * 2 * nbins + 256 * DIAG_NT) * sizeof(double);
-  hipLaunchKernelGGL((k_diag_partial<R>), dim3(DIAG_BLOCKS), dim3(256), shbytes, p->stream, v, p->fstride(), g, nbins,
-                     part);
-  SCL_CHECK(check_launch("diag_partial"));
-  hipLaunchKernelGGL(k_diag_final, dim3((len + 255) / 256), dim3(256), 0, p->stream, part, DIAG_BLOCKS, len, fin);
-  SCL_CHECK(check_launch("diag_final"));
+  double* fin = part + diag_part_len(nbins);
+  SCL_CHECK(diag_launch<R>(p->g, v, p->fstride(), nbins, part, fin, p->stream));
   res.resize(len);
   SCL_HIP(hipMemcpyAsync(res.data(), fin, len * sizeof(double), hipMemcpyDeviceToHost, p->stream));
   SCL_HIP(hipStreamSynchronize(p->stream));
+  return SCL_OK;
+}
+
+// Velocity-derivative skewness <d^3> / <d^2>^(3/2), d = du_x/dx in physical
+// space (skewness.f90:1-48), FP64 sums, of the field vx (split layout).
+template <class R>
+static int32_t skewness_run(scl_plan3d* p, const cpx<R>* vx, double* out) {
+  void* s;
+  SCL_CHECK(plan_scratch(p, 1, &s));
+  cpx<R>* f = (cpx<R>*)s;
+  const Geom& g = p->g;
+  hipLaunchKernelGGL((k_ddx_field<R>), dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, p->stream, vx, f, g);
+  SCL_CHECK(check_launch("ddx_field"));
+  SCL_CHECK(Ops<R>::c2r3d(*p, f, p->fstride(), 1));
+  void* db;
+  SCL_CHECK(plan_diag(p, (size_t)(diag_part_len(g.mx) + 2 * g.mx + DIAG_TOT) * sizeof(double) + 64, &db));
+  double* part = (double*)db;
+  double* sums = part + 2 * DIAG_BLOCKS;
+  hipLaunchKernelGGL((k_moments23<R>), dim3(DIAG_BLOCKS), dim3(256), 0, p->stream, f, g.nmain, part);
+  SCL_CHECK(check_launch("moments23"));
+  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(2), 0, p->stream, part, DIAG_BLOCKS, sums);
+  SCL_CHECK(check_launch("moments_sum"));
+  double h[2];
+  SCL_HIP(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+  SCL_HIP(hipStreamSynchronize(p->stream));
+  const double n = (double)g.nx * g.ny * g.nz;
+  *out = (h[1] / n) / std::pow(h[0] / n, 1.5);
   return SCL_OK;
 }
 
@@ -1034,7 +1055,7 @@ static int32_t divmax_run(scl_plan3d* p, const cpx<R>* v, double* out) {
   SCL_CHECK(check_launch("div_field"));
   SCL_CHECK(Ops<R>::c2r3d(*p, f, p->fstride(), 1));
   void* db;
-  SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * (2 * (g.mx) + DIAG_TOT) * sizeof(double) + 64, &db));
+  SCL_CHECK(plan_diag(p, (size_t)(diag_part_len(g.mx) + 2 * g.mx + DIAG_TOT) * sizeof(double) + 64, &db));
   unsigned long long* mx = (unsigned long long*)((char*)db + p->diag_bytes - 64);
   SCL_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long), p->stream));
   hipLaunchKernelGGL((k_absmax<R>), dim3(256), dim3(256), 0, p->stream, f, g.nmain, mx);
@@ -1051,7 +1072,7 @@ static int32_t divmax_run(scl_plan3d* p, const cpx<R>* v, double* out) {
 template <class R>
 static int32_t scalar_diag_run(scl_plan3d* p, const cpx<R>* phi, double* var, double* kvar) {
   void* buf;
-  SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * (2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &buf));
+  SCL_CHECK(plan_diag(p, (size_t)(diag_part_len(p->g.mx) + 2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &buf));
   double* part = (double*)buf;
   double* sums = part + 2 * DIAG_BLOCKS;
   hipLaunchKernelGGL((k_scalar_diag<R>), dim3(DIAG_BLOCKS), dim3(256), 0, p->stream, phi, p->g, part);
@@ -1117,6 +1138,7 @@ extern "C" int32_t scl_dns_diagnostics(scl_dns* d, scl_diag* out) {
     double c[8];
     SCL_HIP(hipMemcpy(c, d->coef, sizeof(c), hipMemcpyDeviceToHost));
     out->cs2 = d->p.sgs ? c[5] : 0.0;
+    SCL_CHECK(skewness_run<R>(p, v, &out->skewness));
     SCL_CHECK(divmax_run<R>(p, v, &out->divmax));
     // every field of *out is filled; the status reports a non-finite state
     const int32_t nf = read_nonfinite(d);
@@ -1223,7 +1245,7 @@ extern "C" int32_t scl_dynsmag(scl_plan3d* p, const void* ux, const void* uy, co
                          dim3(256), 0, p->stream, gref, gsplit, p->g);
       SCL_CHECK(check_launch("ref_to_split_real"));
       void* part;
-      SCL_CHECK(plan_diag(p, (size_t)(DIAG_BLOCKS + 1) * (2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &part));
+      SCL_CHECK(plan_diag(p, (size_t)(diag_part_len(p->g.mx) + 2 * p->g.mx + DIAG_TOT) * sizeof(double) + 64, &part));
       SCL_CHECK(ExtOps<R>::dyn_update(*p, V, D, fs, gsplit, (R)delta, (R)delta_test, p->coef, (double*)part));
       // tau = -cs2 * FFT(2 D^2 |S| S_ij)/N^3 (dynsmag.f90:186-193), from D[0..5]
       hipLaunchKernelGGL((k_tau_from_s<R>), ExtOps<R>::grid_all(p->g), dim3(256), 0, p->stream, D, fs, p->g,
@@ -1270,6 +1292,24 @@ extern "C" int32_t scl_convecphi_dns(scl_plan3d* p, const void* vx, const void* 
     hipLaunchKernelGGL((k_divflux<R>), ExtOps<R>::grid_all(g), dim3(256), 0, p->stream, W, fs, g);
     SCL_CHECK(check_launch("divflux"));
     return download<R>(p, W, wphi);
+  });
+}
+
+extern "C" int32_t scl_skewness(scl_plan3d* p, const void* ux, const void* kx, double* s, int32_t nx, int32_t ny,
+                                int32_t nz) {
+  SCL_PLAN(p);
+  if (nx != p->g.nx || ny != p->g.ny || nz != p->g.nz) return set_error("nx,ny,nz do not match the plan"), SCL_EINVAL;
+  if (!s || !kx) return set_error("null kx or s"), SCL_EINVAL;
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    const R* k = (const R*)kx;
+    for (int i = 0; i <= p->g.mx; ++i)
+      if (k[i] != (R)i) return set_error("kx(%d) is not wavenumber()'s", i + 1), SCL_EINVAL;
+    void* w;
+    SCL_CHECK(plan_scratch(p, 2, &w));
+    cpx<R>* u = (cpx<R>*)w + p->fstride();  // the second field: skewness_run uses the first
+    SCL_CHECK(upload<R>(p, ux, u));
+    return skewness_run<R>(p, u, s);
   });
 }
 
