@@ -97,6 +97,8 @@ def convecphi_dns(vx, vy, vz, phi, kx, ky, kz, n):
 @dataclass
 class LESParams:
     nx: int
+    ny: int | None = None  # default nx
+    nz: int | None = None  # default nx
     nu: float = 1e-4
     dt: float = 0.001
     forced: bool = True
@@ -114,6 +116,8 @@ class LESParams:
     def __post_init__(self):
         if self.delta is None:
             self.delta = 2 * np.pi / self.nx
+        self.ny = self.ny or self.nx
+        self.nz = self.nz or self.nx
 
 
 class LESScalar:
@@ -126,8 +130,8 @@ class LESScalar:
     def __init__(self, p: LESParams):
         self.p = p
         n = p.nx
-        self.n = (n, n, n)
-        self.kx, self.ky, self.kz, self.k2 = s3.wavenumber(n, n, n)
+        self.n = (n, p.ny, p.nz)
+        self.kx, self.ky, self.kz, self.k2 = s3.wavenumber(*self.n)
         self.cs2 = p.cs2
         lx = n // 2
         kcut = 2.0 * lx / 3.0

@@ -876,6 +876,7 @@ extern "C" int32_t scl_dns_get_velocity(scl_dns* d, void* vx, void* vy, void* vz
 extern "C" int32_t scl_dns_set_scalar(scl_dns* d, const void* phi) {
   if (!d || d->nc < 4) return set_error("solver has no scalar"), SCL_EINVAL;
   const void* vi[1] = {phi};
+  SCL_CHECK(clear_nonfinite(d));
   return dns_io(d, 3, 1, vi, nullptr);
 }
 

@@ -1,0 +1,155 @@
+"""GPU parity of the kernel instances that the benchmark grids run, on thin
+grids the FP64 oracle finishes in seconds:
+
+* DNS x-rows of 256, 512 and 1024 points (k_row_convec with 128-, 256- and
+  512-point complex rows, whose real-FFT split pairs are 16, 32 and 64 lanes
+  apart): ratesdudt, two host-array IF-RK4 steps and two solver steps
+  (graph replay) against the oracle (dns-deform.f90:160-237, 259-288);
+* the LES + scalar x-pass of the 256^3 run (k_row_ext on 128-point complex
+  rows): the derivs-form RHS and two solver steps against oracle/les_scalar;
+* 2D members at 512^2 and 1024^2 (the x-pass and column kernels of the
+  ensemble's larger groups): the RHS under every column schedule and 20
+  batched steps against oracle/vort2d.
+
+Inputs are white-noise spectra (Hermitian on kx = 0), so every wavenumber
+is exercised.  Bars: 2e-5 of the largest entry (FP32 rounding times log2 of
+the longest line) for RHS and short runs against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import speccodelib_amd as scl
+from oracle import les_scalar as les
+from oracle import spectral3d as orc
+from oracle import vort2d as v2
+
+pytestmark = pytest.mark.gpu
+
+BAR = 2e-5
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def white(shape, seed, k2, nf=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nf):
+        a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / (1.0 + k2)
+        orc.symmetrize3d(a)
+        out.append(a.astype(np.complex64))
+    return out
+
+
+@pytest.mark.parametrize("dims", [(256, 16, 16), (512, 16, 16), (1024, 16, 16), (512, 32, 16)])
+def test_dns_wide_rows(dims):
+    nx, ny, nz = dims
+    print(f"case dns wide rows {dims}", flush=True)
+    p = scl.Plan3D(nx, ny, nz, scl.SP)
+    d = orc.DNS(orc.DNSParams(nx=nx, ny=ny, nz=nz, nu=0.01, dt=0.002, forced=True, fkmax=2.0, eps=0.1))
+    v = white(p.shape, 17, d.k2.astype(np.float64))
+    v64 = [a.astype(np.complex128) for a in v]
+    err = max(relerr(a, b) for a, b in zip(p.ratesdudt(*v, forced=1, fkmax=2.0, eps=0.1), d.rhs(*v64)))
+    print(f"ratesdudt rel err {err:.2e}", flush=True)
+    assert err < BAR
+    w = tuple(v64)
+    for _ in range(2):
+        w = d.rk4_step(*w)
+    u = [a.copy() for a in v]
+    for _ in range(2):
+        p.rk4step(*u, 0.01, 0.002, 1, 2.0, 0.1)
+    err = max(relerr(a, b) for a, b in zip(u, w))
+    print(f"2 host-array IF-RK4 steps rel err {err:.2e}", flush=True)
+    assert err < BAR
+    s = scl.DNS(p, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, use_graph=True)
+    s.set_velocity(*v)
+    s.step(2)
+    s.sync()
+    err = max(relerr(a, b) for a, b in zip(s.get_velocity(), w))
+    print(f"2 solver steps (graph) rel err {err:.2e}", flush=True)
+    assert err < BAR
+    s.close()
+    p.close()
+
+
+@pytest.mark.parametrize("dims", [(256, 16, 32)])
+def test_les_scalar_wide_rows(dims):
+    nx, ny, nz = dims
+    nu, sc, cs2 = 1e-3, 0.7, 0.0289
+    delta = 2 * np.pi / nx
+    print(f"case les+scalar wide rows {dims}", flush=True)
+    prm = les.LESParams(nx=nx, ny=ny, nz=nz, nu=nu, dt=0.002, forced=True, fkmax=2.5, eps=0.1, cs2=cs2,
+                        delta=delta, scalar=True, G=1.0, Sc=sc)
+    o = les.LESScalar(prm)
+    p = scl.Plan3D(nx, ny, nz, scl.SP)
+    s = scl.DNS(p, nu=nu, dt=0.002, forced=1, fkmax=2.5, eps=0.1, sgs=1, cs2=cs2, delta=delta, scalar=True,
+                grad=1.0, schmidt=sc, use_graph=True)
+    k2 = o.k2.astype(np.float64)
+    y = white(p.shape, 23, k2, nf=4)
+    y64 = [a.astype(np.complex128) for a in y]
+    got = s.rhs(*[a.copy() for a in y])
+    want = o.rhs(*[a.copy() for a in y64])
+    k2z = k2.copy()
+    k2z[0, 0, 0] = 0
+    lin = [nu * k2z] * 3 + [nu / sc * k2z]
+    errs = [relerr(a, b - c * x) for a, b, x, c in zip(got, want, y64, lin)]
+    print("rhs rel err (u, v, w, phi)", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < BAR
+    s.set_velocity(*y[:3])
+    s.set_scalar(y[3])
+    s.step(2)
+    s.sync()
+    w = tuple(y64)
+    for _ in range(2):
+        w = o.rk4_step(*w)
+    errs = [relerr(a, b) for a, b in zip((*s.get_velocity(), s.get_scalar()), w)]
+    print("2 solver steps rel err", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < BAR
+    s.close()
+    p.close()
+
+
+KW2 = dict(dt=5e-4, c_hyper=1e-5, n_hyper=8, damp=0.1, fkmin=4.0, fkmax=5.0, eps=0.01)
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_2d_large_members_rhs_every_schedule(n):
+    w = v2.initialize2d(n, -61, 8.0, 0.5)
+    want = v2.Vort2D(v2.Vort2DParams(n=n, **KW2)).rhs(w)
+    e = scl.Ensemble2D([(n, 2)], prec=scl.SP, **KW2)
+    ran = 0
+    for sched in [(8, 8, 0), (8, 16, 0), (16, 8, 0), (16, 16, 1), (8, 8, 1), (16, 8, 1)]:
+        print(f"case ens2d n={n} schedule {sched}", flush=True)
+        try:
+            e.set_schedule(0, *sched)
+        except scl.SclError as x:  # not every schedule fits every line length
+            assert x.status == -1
+            continue
+        err = relerr(e.rhs(0, 1, w.astype(np.complex64)), want)
+        print(f"fp32 rhs rel err {err:.2e}", flush=True)
+        assert err < BAR
+        ran += 1
+    assert ran >= 3
+    e.close()
+
+
+def test_2d_large_members_20_steps():
+    groups = [(512, 2), (1024, 2)]
+    print("case ens2d 512^2 + 1024^2 members, 20 batched steps", flush=True)
+    e = scl.Ensemble2D(groups, prec=scl.SP, **KW2)
+    e.initialize(seed0=-71)
+    e.step(20)
+    q = 0
+    for g, (n, m) in enumerate(groups):
+        o = v2.Vort2D(v2.Vort2DParams(n=n, **KW2))
+        for j in range(m):
+            w = v2.initialize2d(n, -71 - q, 8.0, 0.5)
+            for _ in range(20):
+                w = o.rk4_step(w)
+            err = relerr(e.get_member(g, j), w)
+            print(f"n={n} member {j}: 20 steps rel err {err:.2e}", flush=True)
+            assert err < BAR
+            q += 1
+    e.close()
