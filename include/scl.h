@@ -212,6 +212,20 @@ int32_t scl_dns_spectrum(scl_dns *d, double *ek, int32_t nbins);
 int32_t scl_dns_helicity_spectrum(scl_dns *d, double *hk, int32_t nbins);
 int32_t scl_dns_diagnostics(scl_dns *d, scl_diag *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the spectra and totals of scl_dns_spectrum,
+ * scl_dns_helicity_spectrum and scl_dns_diagnostics, from the same kernels
+ * (bitwise the same values), with no device allocation.  out is a device
+ * array of 2 nbins + 8 doubles the caller owns: E(1..nbins), H(1..nbins),
+ * then sum'|v|^2, sum'k^2|v|^2, sum' 2 Re(w.conj v), sum'|k.v|^2, the count
+ * of non-finite entries, sum'k^2 2 Re(w.conj v), 0, 0.  nbins in [1, nx/2],
+ * at most 512 (the partial sums use a 4 MiB static device array per GPU).
+ * Asynchronous on the plan's stream (no host synchronisation), capturable.
+ * Calls that overlap on one GPU (two streams at once) share that array and
+ * race; serialise them. */
+int32_t scl_dns_spectra_noalloc(scl_dns *d, double *out, int32_t nbins);
+
 /* Per-kernel timing of nsteps steps with HIP events on the plan's stream
  * around every launch (direct launches, also when use_graph is set: event
  * records captured into a graph cannot be timed).  kind_ms[k] /

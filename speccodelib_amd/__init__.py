@@ -142,6 +142,7 @@ def lib():
             "scl_dns_spectrum": ([P, P, I32], I32),
             "scl_dns_helicity_spectrum": ([P, P, I32], I32),
             "scl_dns_diagnostics": ([P, ctypes.POINTER(ScDiag)], I32),
+            "scl_dns_spectra_noalloc": ([P, P, I32], I32),
             "scl_dns_step_timed": ([P, I32, P, P, P], I32),
             "scl_dns_set_scalar": ([P, P], I32),
             "scl_dns_get_scalar": ([P, P], I32),

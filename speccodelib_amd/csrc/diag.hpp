@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_diag_partial(const cpx<R>* __restrict__
 }
 
 // out[j] = sum over the nblocks partials (block order)
-__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ part, int nblocks, int len,
+static __global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ part, int nblocks, int len,
                                                     double* __restrict__ out) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= len) return;
