@@ -48,15 +48,18 @@ KIND_BYTES_F = {
     # z-forward + stage + z-inverse of the next stage state (stage chain):
     # stages 1..3: 14, 17, 17 (W, v, du in; du and W[0..4] out)
     "zfwd_zinv": 16.0,
+    # stage 4 + z-inverse of the new v for the next step: W and du in (3 + 3),
+    # v and W[0..4] out (3 + 5)
+    "zfwd4_zinv": 14.0,
 }
 # The step model that the roofline fraction is priced against: one read +
 # write per axis pass (z-inverse 9, y-inverse 12, x-pass 9, y-forward 6;
 # z-forward + stage 12, 15, 15, 12).
 STEP_BYTES_F = 198.0
-# What these kernels move per step: one z-inverse (8), 4 x (11 + 9 + 6) for
-# the y-inverse, x-pass and y-forward, stages 1..3 fused with the next
-# z-inverse (14 + 17 + 17) and the stage-4 z-forward (9): 169 F.
-IMPL_STEP_BYTES_F = 169.0
+# What these kernels move per step: 4 x (11 + 9 + 6) for the y-inverse,
+# x-pass and y-forward, and the four z-forwards, each fused with the next
+# z-inverse (14 + 17 + 17, and 14 for stage 4 with the next step's): 166 F.
+IMPL_STEP_BYTES_F = 166.0
 
 
 def kind_bytes_f(kinds):
@@ -275,6 +278,19 @@ def bench_les256(scl, steps, n=256):
     }
 
 
+def bench_stream(scl, nbytes=2 << 30, reps=10):
+    """The HBM rate plain streaming kernels reach (the practical ceiling of
+    the passes): read, write and copy of 2 GiB buffers."""
+    out = {"bytes": nbytes, "reps": reps}
+    for kind, name in ((0, "read"), (1, "write"), (2, "copy")):
+        ms, gbps = scl.stream_probe(kind, nbytes, reps)
+        out[name + "_GBps"] = gbps
+        out[name + "_ms"] = ms
+    print(f"[bench] stream probe: read {out['read_GBps']:.0f} GB/s, write {out['write_GBps']:.0f} GB/s, "
+          f"copy {out['copy_GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+    return out
+
+
 def load_traffic(kernel, n):
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
     if os.path.exists(path):
@@ -377,6 +393,7 @@ def main():
     ap.add_argument("--les256", type=int, default=50,
                     help="also time this many steps of the 256^3 LES + scalar (0: skip)")
     ap.add_argument("--no-main", action="store_true", help="skip the 512^3 DNS (profiling runs of the others)")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming HBM probe")
     ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="bytes of a device-to-device copy and a fill run after the timed region (counter "
@@ -393,6 +410,10 @@ def main():
         out = {"metric": METRIC, "value": None, "unit": "steps/s", "note": "512^3 DNS skipped (--no-main)"}
     else:
         out = bench_main(scl, a)
+    out["hbm_stream"] = None if a.no_stream else bench_stream(scl)
+    if out["hbm_stream"] and out.get("roofline"):
+        # the dominant kernel's rate against what a plain copy reaches here
+        out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / out["hbm_stream"]["copy_GBps"]
     out["ens2d"] = None if a.no_ens2d else bench_ens2d(scl, a.ens2d_steps, a.ens2d_member_steps)
     if not a.no_adjoint:
         out["adjoint"] = bench_adjoint(scl)

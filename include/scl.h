@@ -230,12 +230,14 @@ int32_t scl_dns_spectra_noalloc(scl_dns *d, double *out, int32_t nbins);
  * around every launch (direct launches, also when use_graph is set: event
  * records captured into a graph cannot be timed).  kind_ms[k] /
  * kind_count[k] for k < SCL_NKIND, total_ms = event time of the whole run.
- * Within a step the z-forward of stages 1..3 also z-inverses the next stage
- * state (SCL_K_ZFWD_ZINV; stage 4 and every stage of 1024-point z-lines:
- * SCL_K_ZFWD, each stage then with its own SCL_K_ZINV). */
+ * The DNS solver chains its stages: the z-forward of stages 1..3 also
+ * z-inverses the next stage state (SCL_K_ZFWD_ZINV) and that of stage 4 the
+ * new v for the next step (SCL_K_ZFWD4_ZINV), so only the first step after
+ * the state is set runs a z-inverse pass (SCL_K_ZINV) of its own.
+ * SCL_K_ZFWD: unchained z-forwards (host-array steps, LES / scalar). */
 enum {
   SCL_K_FORCE = 0, SCL_K_ZINV = 1, SCL_K_YINV = 2, SCL_K_XPASS = 3, SCL_K_YFWD = 4, SCL_K_ZFWD = 5,
-  SCL_K_HERM = 6, SCL_K_ZFWD_ZINV = 7, SCL_NKIND = 8
+  SCL_K_HERM = 6, SCL_K_ZFWD_ZINV = 7, SCL_K_ZFWD4_ZINV = 8, SCL_NKIND = 9
 };
 int32_t scl_dns_step_timed(scl_dns *d, int32_t nsteps, double *kind_ms, int32_t *kind_count, double *total_ms);
 
@@ -368,6 +370,16 @@ int32_t scl_adjoint_vjp(scl_adjoint *adj, double cs2, const void *sx, const void
                         int32_t on_device);
 /* Event-timed milliseconds of the last forward and backward calls. */
 int32_t scl_adjoint_timing(scl_adjoint *adj, double *forward_ms, double *backward_ms);
+
+/* ---------------------------------------------------------------------
+ * Measurement.  No counterpart in the reference: the HBM rate a plain
+ * streaming kernel reaches on the current device (16-byte vector accesses,
+ * grid-stride, buffers of `bytes` each), the practical ceiling next to the
+ * solver's per-pass rates.  kind 0: read `bytes`; 1: write `bytes`; 2: copy
+ * (read and write `bytes`).  Allocates and frees its own buffers; *ms_per_rep
+ * = event time of one pass, averaged over reps after one warm-up pass.
+ * ------------------------------------------------------------------- */
+int32_t scl_stream_probe(int32_t kind, int64_t bytes, int32_t reps, double *ms_per_rep);
 
 #ifdef __cplusplus
 }

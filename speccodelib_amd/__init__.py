@@ -24,7 +24,8 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "scl.h")
 
 SP, DP = 0, 1
 _STATUS = {0: "SCL_OK", -1: "SCL_EINVAL", -2: "SCL_ENOMEM", -3: "SCL_EHIP", -4: "SCL_ENOPLAN", -5: "SCL_ENONFINITE", -6: "SCL_ENODEV"}
-KIND_NAMES = ["force_coef", "zinv_vort", "y_inverse", "x_pass", "y_forward", "zfwd_rk", "herm_fix", "zfwd_zinv"]
+KIND_NAMES = ["force_coef", "zinv_vort", "y_inverse", "x_pass", "y_forward", "zfwd_rk", "herm_fix", "zfwd_zinv",
+              "zfwd4_zinv"]
 
 _lib = None
 
@@ -177,6 +178,7 @@ def lib():
             "scl_adjoint_backward": ([P] + [P] * 6 + [P, I32], I32),
             "scl_adjoint_vjp": ([P, D] + [P] * 9 + [P, I32], I32),
             "scl_adjoint_timing": ([P, P, P], I32),
+            "scl_stream_probe": ([I32, ctypes.c_int64, I32, P], I32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -605,6 +607,15 @@ class Adjoint:
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().scl_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+
+def stream_probe(kind, nbytes, reps=10):
+    """HBM rate of a plain streaming kernel: kind 0 read, 1 write, 2 copy of
+    nbytes; returns (ms per pass, GB/s of bytes moved: 2 nbytes for a copy)."""
+    ms = ctypes.c_double()
+    _check(lib().scl_stream_probe(kind, nbytes, reps, ctypes.byref(ms)))
+    moved = nbytes * (2 if kind == 2 else 1)
+    return ms.value, moved / (ms.value * 1e-3) / 1e9
 
 
 def ifrk4_integrate(prec, n, y_ptr, rate_ptr, t0, dt, nsteps, fn_ptr, ctx):

@@ -568,7 +568,7 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   SCL_CHECK(Ops<R>::row_convec(*p, W, fs, reinterpret_cast<cpx<R>*>(d.phys_out)));
   tm->mark(SCL_K_YFWD);
   SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
-  tm->mark(fuse ? SCL_K_ZFWD_ZINV : SCL_K_ZFWD);
+  tm->mark(fuse ? (mode == 4 ? SCL_K_ZFWD4_ZINV : SCL_K_ZFWD_ZINV) : SCL_K_ZFWD);
   SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp, fuse));
   // The kx = 0 Hermitian copy (symmetrize.f90:51-61) goes on N (mode 0),
   // on the collected stage states (s_out: the adjoint's tape), and on v once
@@ -601,27 +601,31 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   return SCL_OK;
 }
 
-// Stage chaining within a step: the z-forward of stages 1..3 also
-// z-inverses the next stage state into W[0..4] (k_zfwd_rhs FUSED), so
-// stages 2..4 skip their z-inverse pass and the stage states are stored
-// only on the forced modes.  Each step starts from v (its own z-inverse
-// pass) and leaves no state in W, so a step's result does not depend on
-// how steps are grouped into calls.  Not when the stage states are
-// collected (s_out: the adjoint's tape).
+// Stage chaining: the z-forward of stages 1..3 also z-inverses the next
+// stage state into W[0..4] (k_zfwd_rhs FUSED), so stages 2..4 skip their
+// z-inverse pass and the stage states are stored only on the forced modes.
+// Not when the stage states are collected (s_out: the adjoint's tape).
 template <class R>
 static bool chain_stages(const scl_plan3d*, const DevFields& d) {
   return !d.s_out;
 }
 
+// One IF-RK4 step.  primed: W[0..4] already hold the z-inverse of v (the
+// previous step's fused stage 4), so stage 1 runs no z-inverse pass;
+// fuse_last: stage 4 z-inverses the new v into W[0..4] for the next step.
+// The device solver sets both from its first step on, so every step after
+// the first reads the fused z-inverse whatever the grouping of steps into
+// calls or graphs: results do not depend on it.
 template <class R>
-static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm) {
+static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm,
+                            bool primed = false, bool fuse_last = false) {
   RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
   rp.nonfinite = d.nonfinite;
   const bool chain = chain_stages<R>(p, d);
   for (int stage = 1; stage <= 4; ++stage) {
     const cpx<R>* st = stage == 1 ? d.V<R>() : d.S<R>();
-    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm, chain && stage > 1,
-                          chain && stage < 4));
+    SCL_CHECK(rk_stage<R>(p, d, stage, rp, prm.eps, prm.eta, prm.fkmax, st, tm,
+                          chain && (stage > 1 || primed), chain && (stage < 4 || fuse_last)));
   }
   return SCL_OK;
 }
@@ -864,6 +868,7 @@ extern "C" int32_t scl_dns_set_velocity(scl_dns* d, const void* vx, const void* 
   if (!d) return set_error("null solver"), SCL_EINVAL;
   const void* vi[3] = {vx, vy, vz};
   SCL_CHECK(clear_nonfinite(d));
+  d->chained = false;
   return dns_io(d, 0, 3, vi, nullptr);
 }
 
@@ -904,7 +909,13 @@ template <class R>
 static int32_t dns_step_once(scl_dns* d, Timer* tm) {
   scl_plan3d* p = d->plan;
   DevFields f = dev_fields(d);
-  if (!d->ext) return rk4_step_dev<R>(p, f, d->p, tm);
+  if (!d->ext) {
+    const bool primed = d->chained;
+    d->chained = false;  // until the step is queued whole
+    SCL_CHECK(rk4_step_dev<R>(p, f, d->p, tm, primed, true));
+    d->chained = true;
+    return SCL_OK;
+  }
   const scl_dns_params& prm = d->p;
   if (prm.sgs == 2 && d->steps % (prm.iupdate > 0 ? prm.iupdate : 1) == 0) {
     const double delta = prm.delta > 0 ? prm.delta : 2.0 * M_PI / p->g.nx;
@@ -923,11 +934,19 @@ template <class R>
 static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
   scl_plan3d* p = d->plan;
   if (d->p.use_graph && !tm) {
+    if (!d->ext && !d->chained) {
+      // the first step after the state was set: its z-inverse pass runs
+      // here, so that the captured step is the chained one
+      DevFields f = dev_fields(d);
+      SCL_CHECK(Ops<R>::zinv_vort(*p, f.V<R>(), f.fs, f.W<R>(), f.fs));
+      d->chained = true;
+    }
     if (!d->exec) {
       SCL_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
       const int64_t keep = d->steps;
       int32_t st = dns_step_once<R>(d, nullptr);
       d->steps = keep;
+      if (st != SCL_OK) d->chained = false;
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(p->stream, &gr);
       if (st != SCL_OK) {

@@ -102,6 +102,7 @@ int32_t staging_of(scl_plan3d* p, cpx<R>** st) {
 // fields), st = S of the solver.
 template <class R>
 int32_t rhs_split(scl_dns* d, const cpx<R>* st, cpx<R>* out, bool linear) {
+  d->chained = false;  // W is overwritten
   scl_plan3d* p = d->plan;
   DevFields f{d->fields, p->fstride(), d->nc, d->coef};
   SCL_CHECK(solver_stage<R>(p, f, 0, d->p, st));
@@ -162,6 +163,7 @@ extern "C" int32_t scl_dns_state(scl_dns* d, void** state, int64_t* n) {
   if (!d || !state || !n) return set_error("null argument"), SCL_EINVAL;
   *state = d->fields;
   *n = (int64_t)d->nc * d->plan->fstride();
+  d->chained = false;  // the caller may change the state through the pointer
   return SCL_OK;
 }
 

@@ -382,11 +382,11 @@ struct RhsParams {
 //   MODE 4: v = du + a/6, 0 where k2 >= kc2
 // so the last stage reads no v, and no stage divides by E.
 //
-// FUSED (MODE 1..3): the new stage state stays in registers and its
-// z-inverse goes to W[0..4] for the next RHS (zinv_store), so that RHS
-// needs no z-inverse pass of its own; S is then stored only where the
-// forcing reads it (k2 < fk2).  Each workgroup reads all of its columns of
-// W before it writes them.
+// FUSED (MODE 1..4): the new stage state (MODE 4: the new v) stays in
+// registers and its z-inverse goes to W[0..4] for the next RHS
+// (zinv_store), so that RHS needs no z-inverse pass of its own; S is then
+// stored only where the forcing reads it (k2 < fk2).  Each workgroup reads
+// all of its columns of W before it writes them.
 template <class R, int N, int MODE, int CWZ, int EZ = 0, bool FUSED = false, int ZS = 0>
 __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>* W, int64_t fstride, cpx<R>* V,
                                                                     cpx<R>* DU, cpx<R>* S, const cpx<R>* st,

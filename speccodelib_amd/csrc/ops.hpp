@@ -19,8 +19,8 @@ struct Ops {
   static int32_t colfft(const scl_plan3d& pl, C* f0, int64_t fstride, int nf, int axis, int dir);
   static int32_t zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* w, int64_t wstride);
   static int32_t yinv_vort(const scl_plan3d& pl, C* W, int64_t fstride);
-  // fused (mode 1..3): also z-inverse the new stage state into W[0..4] for
-  // the next RHS (k_zfwd_rhs FUSED)
+  // fused (mode 1..4): also z-inverse the new stage state (mode 4: the new
+  // v) into W[0..4] for the next RHS (k_zfwd_rhs FUSED)
   static int32_t zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, C* V, C* DU, C* S, const C* st,
                           const double* coef, const RhsParams<R>& p, bool fused = false);
   // the LES / scalar RHS's z-passes fused with its prep and combine

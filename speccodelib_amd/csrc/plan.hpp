@@ -59,6 +59,10 @@ struct scl_dns {
                              // sticky non-finite flag (an unsigned at coef + 8)
   double time = 0.0;
   int64_t steps = 0;
+  // W[0..4] hold the z-inverse of the current v (u and w_x, w_y, left there
+  // by the last step's fused stage 4): the next step's stage 1 needs no
+  // z-inverse pass.  Cleared whenever v or W change outside a step.
+  bool chained = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipEvent_t> events;

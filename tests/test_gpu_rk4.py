@@ -81,6 +81,41 @@ def test_graph_replay_is_bitwise_equal_to_eager():
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("n", [32, 64])
+def test_steps_do_not_depend_on_grouping(n):
+    """The stage chain carries the z-inverse of v from one step to the next
+    (stage 4's fused z-pass): 6 steps in one call, in calls of 2, 1 and 3
+    (graph and eager, with diagnostics and get_velocity in between) and in a
+    timed run must give the same bits; setting the state restarts the chain
+    from a z-inverse pass."""
+    print(f"case step grouping n={n}", flush=True)
+    p = scl.Plan3D(n, n, n, scl.SP)
+    v0 = scl.initialize(n, n, n, 1, 4.0, 1.0, scl.SP)
+    runs = []
+    for g, groups in ((True, [6]), (True, [2, 1, 3]), (False, [6]), (False, [1, 1, 4]), (None, [3, 3])):
+        s = scl.DNS(p, nu=0.01, dt=0.002, forced=1, use_graph=bool(g))
+        s.set_velocity(*[a.copy() for a in v0])
+        for k in groups:
+            if g is None:
+                s.step_timed(k)
+            else:
+                s.step(k)
+            s.sync()
+            _ = s.spectrum(), s.diagnostics(), s.get_velocity()
+        runs.append(s.get_velocity())
+        # a new state restarts the chain: the same 6 steps from v0 again
+        s.set_velocity(*[a.copy() for a in v0])
+        s.step(6)
+        s.sync()
+        for a, b in zip(runs[-1], s.get_velocity()):
+            np.testing.assert_array_equal(a, b)
+        s.close()
+    for r in runs[1:]:
+        for a, b in zip(runs[0], r):
+            np.testing.assert_array_equal(a, b)
+    p.close()
+
+
 def test_helical_forcing_solver_fp64():
     n = 32
     nu, dt, eps, eta = 0.01, 0.002, 0.1, 0.05
