@@ -175,6 +175,22 @@ typedef struct scl_dns_params {
   int32_t scalar;      /* 1: advance phi with -div(u phi) - grad * u_z, diffusivity nu/schmidt */
   double grad;         /* mean scalar gradient along z */
   double schmidt;      /* Sc (Prandtl number pr of the reference) */
+  /* time stepper (DNS without LES or scalar for steppers 1 and 2) */
+  int32_t stepper;     /* 0 Lawson IF-RK4 (dns-deform.f90:160-237);
+                          1 IF-AB2 with CFL steps (spectral_dns_dealiased.f90:124-296):
+                            an Euler step, a leapfrog step, then variable-step
+                            Adams-Bashforth, dt computed on the device;
+                          2 IF-AB2 with the fixed dt of .dt and a half-step bootstrap
+                            (specdns-forced-omp-ft.f90:224-274) */
+  double beta;         /* stepper 1: dt = min(beta (pi/lx) / umax, 3 dto) */
+  double dtw;          /* stepper 1: steps end on the output times (twrite, advanced by
+                          dtw each time it is reached); <= 0: no landing */
+  int32_t ieout;       /* stepper 1: umax = 3.5 sqrt(2 E) at steps that are multiples of
+                          ieout; <= 0: never */
+  double umax;         /* stepper 1: umax until the first refresh (3 sqrt(3) u0 in the reference) */
+  double dt_first;     /* stepper 1: the first step's dto (timemax in the reference); <= 0: none */
+  double kcut;         /* truncation after each step, k^2 >= kcut^2 zeroed; <= 0: 2 lx/3
+                          (spectral_dns_dealiased.f90 uses lx - 1) */
 } scl_dns_params;
 
 typedef struct scl_diag {
@@ -203,7 +219,11 @@ int32_t scl_dns_initialize(scl_dns *d, int32_t iseed, double akp, double u0);
 /* nsteps IF-RK4 steps, asynchronous on the plan's stream */
 int32_t scl_dns_step(scl_dns *d, int32_t nsteps);
 int32_t scl_dns_sync(scl_dns *d);
+/* The solver's time (IF-AB2: read from the device after the queued steps). */
 double scl_dns_time(const scl_dns *d);
+/* Time, the last step's dt and the steps taken (IF-AB2: since the state was
+ * set), after the queued steps. */
+int32_t scl_dns_step_info(scl_dns *d, double *time, double *dt_last, int64_t *steps);
 /* E(ll), ll = 1..nbins (nbins <= nx/2), FP64 (spec.f90:66-89).  The
  * diagnostics sum in an order fixed by the grid: repeated calls on one state
  * give the same bits. */

@@ -59,6 +59,13 @@ class ScDnsParams(ctypes.Structure):
         ("scalar", ctypes.c_int32),
         ("grad", ctypes.c_double),
         ("schmidt", ctypes.c_double),
+        ("stepper", ctypes.c_int32),
+        ("beta", ctypes.c_double),
+        ("dtw", ctypes.c_double),
+        ("ieout", ctypes.c_int32),
+        ("umax", ctypes.c_double),
+        ("dt_first", ctypes.c_double),
+        ("kcut", ctypes.c_double),
     ]
 
 
@@ -140,6 +147,7 @@ def lib():
             "scl_dns_step": ([P, I32], I32),
             "scl_dns_sync": ([P], I32),
             "scl_dns_time": ([P], D),
+            "scl_dns_step_info": ([P, P, P, P], I32),
             "scl_dns_spectrum": ([P, P, I32], I32),
             "scl_dns_helicity_spectrum": ([P, P, I32], I32),
             "scl_dns_diagnostics": ([P, ctypes.POINTER(ScDiag)], I32),
@@ -358,10 +366,16 @@ class DNS:
 
     def __init__(self, plan: Plan3D, nu=0.01, dt=0.002, forced=1, fkmax=2.0, eps=0.1, eta=0.0, dealias_rhs=False,
                  use_graph=True, sgs=0, cs2=0.0, delta=0.0, delta_test=0.0, k_test=0.0, iupdate=10, scalar=False,
-                 grad=0.0, schmidt=1.0):
+                 grad=0.0, schmidt=1.0, stepper=0, beta=0.5, dtw=0.0, ieout=10, umax=3.0 * 3.0**0.5, dt_first=0.0,
+                 kcut=0.0):
+        """stepper: 0 IF-RK4 (dns-deform.f90), 1 IF-AB2 with CFL steps
+        (spectral_dns_dealiased.f90; beta, dtw, ieout, umax, dt_first), 2
+        IF-AB2 with fixed dt (specdns-forced-omp-ft.f90).  kcut: truncation
+        radius after each step (0: 2 lx/3)."""
         self.plan = plan
         self.params = ScDnsParams(nu, dt, int(forced), fkmax, eps, eta, int(dealias_rhs), int(use_graph), int(sgs), cs2,
-                                  delta, delta_test, k_test, int(iupdate), int(scalar), grad, schmidt)
+                                  delta, delta_test, k_test, int(iupdate), int(scalar), grad, schmidt, int(stepper),
+                                  beta, dtw, int(ieout), umax, dt_first, kcut)
         h = ctypes.c_void_p()
         _check(lib().scl_dns_create(plan.h, ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
@@ -404,6 +418,12 @@ class DNS:
     @property
     def time(self):
         return lib().scl_dns_time(self.h)
+
+    def step_info(self):
+        """(time, the last step's dt, steps since the state was set)."""
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _check(lib().scl_dns_step_info(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)))
+        return t.value, dt.value, n.value
 
     def spectrum(self):
         ek = np.zeros(self.plan.nx // 2)

@@ -568,7 +568,7 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   SCL_CHECK(Ops<R>::row_convec(*p, W, fs, reinterpret_cast<cpx<R>*>(d.phys_out)));
   tm->mark(SCL_K_YFWD);
   SCL_CHECK(Ops<R>::colfft(*p, W, fs, 3, 1, -1));
-  tm->mark(fuse ? (mode == 4 ? SCL_K_ZFWD4_ZINV : SCL_K_ZFWD_ZINV) : SCL_K_ZFWD);
+  tm->mark(fuse ? (mode >= 4 ? SCL_K_ZFWD4_ZINV : SCL_K_ZFWD_ZINV) : SCL_K_ZFWD);
   SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp, fuse));
   // The kx = 0 Hermitian copy (symmetrize.f90:51-61) goes on N (mode 0),
   // on the collected stage states (s_out: the adjoint's tape), and on v once
@@ -616,10 +616,18 @@ static bool chain_stages(const scl_plan3d*, const DevFields& d) {
 // The device solver sets both from its first step on, so every step after
 // the first reads the fused z-inverse whatever the grouping of steps into
 // calls or graphs: results do not depend on it.
+// The truncation radius after a step: kcut > 0 from the parameters, else
+// dns-deform.f90:75-76's 2 lx/3 (rhs_params).
+template <class R>
+static void apply_kcut(RhsParams<R>& rp, const scl_dns_params& prm) {
+  if (prm.kcut > 0) rp.kc2 = (R)prm.kcut * (R)prm.kcut;
+}
+
 template <class R>
 static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, Timer* tm,
                             bool primed = false, bool fuse_last = false) {
   RhsParams<R> rp = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  apply_kcut<R>(rp, prm);
   rp.nonfinite = d.nonfinite;
   const bool chain = chain_stages<R>(p, d);
   for (int stage = 1; stage <= 4; ++stage) {
@@ -628,6 +636,44 @@ static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_par
                           chain && (stage > 1 || primed), chain && (stage < 4 || fuse_last)));
   }
   return SCL_OK;
+}
+
+// One IF-AB2 step (spectral_dns_dealiased.f90:124-296 / specdns-forced-
+// omp-ft.f90:224-274) of the DNS solver: the umax energy (device-guarded),
+// dt and the weights (k_ab_begin), the RHS passes of one stage with the
+// z-forward epilogue of mode 5 (Euler), 6 (leapfrog) or 7 (Adams-Bashforth),
+// fused with the z-inverse of the new v for the next step, then the kx = 0
+// Hermitian copy on v and the time update (k_ab_end).  istep: steps since
+// the state was set.
+template <class R>
+static int32_t ab2_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, double* ab,
+                            double* ab_part, int64_t istep, bool primed, Timer* tm) {
+  const Geom& g = p->g;
+  RhsParams<R> rp = rhs_params<R>(p, prm.nu, 0.0, prm.forced, prm.fkmax, 0);
+  apply_kcut<R>(rp, prm);
+  rp.nonfinite = d.nonfinite;
+  rp.ab = ab;
+  AbConsts c;
+  c.stepper = prm.stepper;
+  c.ieout = prm.ieout;
+  c.betadx = prm.beta * M_PI / g.mx;  // beta (pi/lx), spectral_dns_dealiased.f90:74-75
+  c.dtw = prm.dtw;
+  c.myeps = 2.0 * (double)std::numeric_limits<float>::epsilon();
+  c.dt_fixed = prm.dt;
+  Timer none;
+  if (!tm) tm = &none;
+  tm->mark(-1);
+  if (prm.stepper == 1) {
+    hipLaunchKernelGGL((k_ab_energy<R>), dim3(DIAG_BLOCKS), dim3(256), 0, p->stream, d.V<R>(), d.fs, g,
+                       (const double*)ab, prm.ieout, ab_part);
+    SCL_CHECK(check_launch("ab_energy"));
+  }
+  hipLaunchKernelGGL(k_ab_begin, dim3(1), dim3(64), 0, p->stream, ab, (const double*)ab_part, DIAG_BLOCKS, c);
+  SCL_CHECK(check_launch("ab_begin"));
+  const int mode = istep == 0 ? 5 : istep == 1 ? 6 : 7;
+  SCL_CHECK(rk_stage<R>(p, d, mode, rp, prm.eps, prm.eta, prm.fkmax, d.V<R>(), tm, primed, true));
+  hipLaunchKernelGGL(k_ab_end, dim3(1), dim3(64), 0, p->stream, ab);
+  return check_launch("ab_end");
 }
 
 // The LES and/or scalar RHS (multi-pass) for stage `mode`.
@@ -776,7 +822,27 @@ static void free_dns(scl_dns* d) {
   if (d->gt) (void)hipFree(d->gt);
   if (d->part) (void)hipFree(d->part);
   if (d->coef) (void)hipFree(d->coef);
+  if (d->ab) (void)hipFree(d->ab);
+  if (d->ab_part) (void)hipFree(d->ab_part);
   delete d;
+}
+
+// IF-AB2: a new state restarts the bootstrap at the current time.
+static int32_t ab_reset(scl_dns* d) {
+  if (!d->ab) return SCL_OK;
+  double h[AB_NSLOT];
+  SCL_HIP(hipMemcpyAsync(h, d->ab, sizeof(h), hipMemcpyDeviceToHost, d->plan->stream));
+  SCL_HIP(hipStreamSynchronize(d->plan->stream));
+  const double t = h[AB_TIME];
+  for (double& x : h) x = 0.0;
+  h[AB_TIME] = t;
+  h[AB_DTO] = d->p.dt_first > 0 ? d->p.dt_first : 1.0e30;
+  h[AB_UMAX] = d->p.umax;
+  h[AB_TWRITE] = t;
+  SCL_HIP(hipMemcpyAsync(d->ab, h, sizeof(h), hipMemcpyHostToDevice, d->plan->stream));
+  SCL_HIP(hipStreamSynchronize(d->plan->stream));  // h is on the host stack
+  d->ab_steps = 0;
+  return SCL_OK;
 }
 
 extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_dns** out) {
@@ -785,6 +851,12 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
   SCL_PLAN(p);
   if (prm->forced < 0 || prm->forced > 2) return set_error("forced must be 0, 1 or 2"), SCL_EINVAL;
   if (prm->sgs < 0 || prm->sgs > 2) return set_error("sgs must be 0, 1 or 2"), SCL_EINVAL;
+  if (prm->stepper < 0 || prm->stepper > 2) return set_error("stepper must be 0, 1 or 2"), SCL_EINVAL;
+  if (prm->stepper && (prm->sgs || prm->scalar))
+    return set_error("the IF-AB2 steppers take the DNS without LES or scalar"), SCL_EINVAL;
+  if (prm->stepper == 1 && !(prm->beta > 0 && prm->umax > 0))
+    return set_error("stepper 1 needs beta > 0 and umax > 0"), SCL_EINVAL;
+  if (prm->stepper == 2 && !(prm->dt > 0)) return set_error("stepper 2 needs dt > 0"), SCL_EINVAL;
   if (prm->scalar && !(prm->schmidt > 0)) return set_error("scalar needs schmidt > 0"), SCL_EINVAL;
   auto* d = new scl_dns();
   d->plan = p;
@@ -807,6 +879,11 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
   };
   if (hipMalloc(&d->fields, bytes) != hipSuccess) return fail(bytes);
   if (hipMalloc((void**)&d->coef, 16 * sizeof(double)) != hipSuccess) return fail(128);
+  if (prm->stepper) {
+    if (hipMalloc((void**)&d->ab, AB_NSLOT * sizeof(double)) != hipSuccess) return fail(AB_NSLOT * 8);
+    if (hipMalloc((void**)&d->ab_part, DIAG_BLOCKS * sizeof(double)) != hipSuccess) return fail(DIAG_BLOCKS * 8);
+    if (hipMemsetAsync(d->ab, 0, AB_NSLOT * sizeof(double), p->stream) != hipSuccess) return fail(0);
+  }
   if (prm->sgs == 2) {
     if (hipMalloc(&d->dyn, 24 * fb) != hipSuccess) return fail(24 * fb);
     if (hipMalloc(&d->gt, (size_t)p->fstride() * (p->esize() / 2)) != hipSuccess) return fail(fb / 2);
@@ -836,6 +913,10 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
   if (hipStreamSynchronize(p->stream) != hipSuccess) {
     free_dns(d);
     return set_error("solver initialisation failed"), SCL_EHIP;
+  }
+  if (const int32_t st = ab_reset(d); st != SCL_OK) {
+    free_dns(d);
+    return st;
   }
   *out = d;
   return SCL_OK;
@@ -869,6 +950,7 @@ extern "C" int32_t scl_dns_set_velocity(scl_dns* d, const void* vx, const void* 
   const void* vi[3] = {vx, vy, vz};
   SCL_CHECK(clear_nonfinite(d));
   d->chained = false;
+  SCL_CHECK(ab_reset(d));
   return dns_io(d, 0, 3, vi, nullptr);
 }
 
@@ -912,7 +994,12 @@ static int32_t dns_step_once(scl_dns* d, Timer* tm) {
   if (!d->ext) {
     const bool primed = d->chained;
     d->chained = false;  // until the step is queued whole
-    SCL_CHECK(rk4_step_dev<R>(p, f, d->p, tm, primed, true));
+    if (d->p.stepper) {
+      SCL_CHECK(ab2_step_dev<R>(p, f, d->p, d->ab, d->ab_part, d->ab_steps, primed, tm));
+      d->ab_steps += 1;
+    } else {
+      SCL_CHECK(rk4_step_dev<R>(p, f, d->p, tm, primed, true));
+    }
     d->chained = true;
     return SCL_OK;
   }
@@ -934,6 +1021,14 @@ template <class R>
 static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
   scl_plan3d* p = d->plan;
   if (d->p.use_graph && !tm) {
+    // IF-AB2: the two bootstrap steps run eagerly; the captured step is the
+    // Adams-Bashforth one
+    while (d->p.stepper && d->ab_steps < 2 && nsteps > 0) {
+      SCL_CHECK(dns_step_once<R>(d, nullptr));
+      d->steps += 1;
+      --nsteps;
+    }
+    if (nsteps == 0) return SCL_OK;
     if (!d->ext && !d->chained) {
       // the first step after the state was set: its z-inverse pass runs
       // here, so that the captured step is the chained one
@@ -943,9 +1038,10 @@ static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
     }
     if (!d->exec) {
       SCL_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-      const int64_t keep = d->steps;
+      const int64_t keep = d->steps, keep_ab = d->ab_steps;
       int32_t st = dns_step_once<R>(d, nullptr);
       d->steps = keep;
+      d->ab_steps = keep_ab;
       if (st != SCL_OK) d->chained = false;
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(p->stream, &gr);
@@ -960,6 +1056,7 @@ static int32_t dns_launch_steps(scl_dns* d, int nsteps, Timer* tm) {
     for (int i = 0; i < nsteps; ++i) {
       SCL_HIP(hipGraphLaunch(d->exec, p->stream));
       d->steps += 1;
+      if (d->p.stepper) d->ab_steps += 1;
     }
   } else {
     for (int i = 0; i < nsteps; ++i) {
@@ -985,7 +1082,35 @@ extern "C" int32_t scl_dns_sync(scl_dns* d) {
   return read_nonfinite(d);
 }
 
-extern "C" double scl_dns_time(const scl_dns* d) { return d ? d->time : 0.0; }
+extern "C" double scl_dns_time(const scl_dns* d) {
+  if (!d) return 0.0;
+  if (!d->ab) return d->time;
+  double t = 0.0;
+  if (hipMemcpyAsync(&t, d->ab + AB_TIME, sizeof(t), hipMemcpyDeviceToHost, d->plan->stream) != hipSuccess ||
+      hipStreamSynchronize(d->plan->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return std::numeric_limits<double>::quiet_NaN();
+  }
+  return t;
+}
+
+extern "C" int32_t scl_dns_step_info(scl_dns* d, double* time, double* dt_last, int64_t* steps) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  double t = d->time, dt = d->p.dt;
+  if (d->ab) {
+    double h[AB_NSLOT];
+    SCL_HIP(hipMemcpyAsync(h, d->ab, sizeof(h), hipMemcpyDeviceToHost, d->plan->stream));
+    SCL_HIP(hipStreamSynchronize(d->plan->stream));
+    t = h[AB_TIME];
+    dt = h[AB_DT];
+  } else {
+    SCL_HIP(hipStreamSynchronize(d->plan->stream));
+  }
+  if (time) *time = t;
+  if (dt_last) *dt_last = dt;
+  if (steps) *steps = d->ab ? d->ab_steps : d->steps;
+  return SCL_OK;
+}
 
 extern "C" int32_t scl_dns_step_timed(scl_dns* d, int32_t nsteps, double* kind_ms, int32_t* kind_count,
                                       double* total_ms) {

@@ -365,7 +365,44 @@ struct RhsParams {
   int forced;   // 0 none, 1 force(), 2 force_heli()
   int dealias;  // 1: zero k2 >= kc2 in the RHS as well
   unsigned* nonfinite = nullptr;  // sticky flag set by stage 4 on a non-finite v (null: no check)
+  const double* ab = nullptr;     // IF-AB2 (MODE 5..7): the step's coefficients on the device (AB_*)
 };
+
+// Slots of the IF-AB2 device state (scl_dns::ab): time, this step's dt, dto,
+// umax, twrite, the step count, and the weights of the update.
+enum { AB_TIME = 0, AB_DT = 1, AB_DTO = 2, AB_UMAX = 3, AB_TWRITE = 4, AB_ISTEP = 5, AB_S = 6, AB_EK = 7,
+       AB_DELTA = 8, AB_NSLOT = 16 };
+
+// One IF-AB2 update of one component (spectral_dns_dealiased.f90:237-285),
+// a = N (symmetrized, projected), with e1 = expm1(-nu k2 dt) and
+// e2 = expm1(-nu k2 dto), so E w = w + e w exactly to the rounding of the
+// result:
+//   MODE 5 (step 0):  v1 = (v0 + dt a) E(dt);  DU = v0, S = a (N0)
+//   MODE 6 (step 1):  v2 = [cS E(dto) v0 + cek v1 + cdel a] E(dt)  (v0 from DU)
+//   MODE 7 (>= 2):    v = [v + cS a + cek E(dto) O] E(dt);  S = a (O = S)
+// The new v is 0 where cut (k2 >= kcut^2, and the mean).  nf: as stage_keep.
+template <int MODE, class R, class I>
+__device__ inline cpx<R> ab_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS, R cek, R cdel, R e1,
+                                 R e2, bool cut, unsigned* nf) {
+  const cpx<R> v = V[o];
+  cpx<R> y;
+  if constexpr (MODE == 5) {
+    DU[o] = v;
+    S[o] = a;
+    y = v + dt * a;
+  } else if constexpr (MODE == 6) {
+    const cpx<R> v0 = DU[o];
+    y = cS * (v0 + e2 * v0) + cek * v + cdel * a;
+  } else {
+    const cpx<R> ob = S[o];
+    y = v + cS * a + cek * (ob + e2 * ob);
+    S[o] = a;
+  }
+  const cpx<R> out = cut ? mk<R>(0, 0) : y + e1 * y;
+  V[o] = out;
+  if (nf && !(isfinite(out.x) && isfinite(out.y)) && *(volatile unsigned*)nf == 0u) atomicOr(nf, 1u);
+  return out;
+}
 
 // ZFWD: z-forward of the three transformed products + forcing + the
 // pointwise part of symmetrize + projection (+ optional RHS truncation),
@@ -443,6 +480,14 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     fa = (R)coef[0];
     fb = (R)coef[1];
   }
+  R ab_dt = 0, ab_dto = 0, ab_s = 0, ab_ek = 0, ab_del = 0;
+  if constexpr (MODE >= 5) {
+    ab_dt = (R)p.ab[AB_DT];
+    ab_dto = (R)p.ab[AB_DTO];
+    ab_s = (R)p.ab[AB_S];
+    ab_ek = (R)p.ab[AB_EK];
+    ab_del = (R)p.ab[AB_DELTA];
+  }
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int z = L::out_pos(m, rt);
@@ -487,6 +532,16 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       W[o] = a;
       W1[o] = b;
       W2[o] = d;
+    } else if constexpr (MODE >= 5) {
+      const R xk = -p.nu * k2;
+      const R e1 = expm1_r(xk * ab_dt);
+      const R e2 = MODE == 5 ? (R)0 : expm1_r(xk * ab_dto);
+      const bool cut = k2 >= p.kc2 || ik2 == 0;
+      nx_[m] = ab_keep<MODE, R>(a, o, V, DU, S, ab_dt, ab_s, ab_ek, ab_del, e1, e2, cut, p.nonfinite);
+      ny_[m] = ab_keep<MODE, R>(b, o, V + fstride, DU + fstride, S + fstride, ab_dt, ab_s, ab_ek, ab_del, e1, e2,
+                                cut, p.nonfinite);
+      nz_[m] = ab_keep<MODE, R>(d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, ab_dt, ab_s, ab_ek, ab_del,
+                                e1, e2, cut, p.nonfinite);
     } else {
       // E = exp(x), x = -nu k2 dt/2.  E is within 1e-3 of 1 for most modes,
       // where FP32 spacing (6e-8) makes a rounded E or E*E a biased factor
@@ -578,6 +633,96 @@ __global__ __launch_bounds__(256) void k_force_coef(const cpx<R>* __restrict__ s
     coef[3] = O;
     coef[4] = U;
   }
+}
+
+// IF-AB2 energy of v for the umax refresh (spectral_dns_dealiased.f90:
+// 129-142: sum' |v|^2, half weight on kx = 0), per-workgroup FP64 partials;
+// only on steps that refresh umax (the step count is read on the device,
+// so a captured step decides by itself).
+template <class R>
+__global__ __launch_bounds__(256) void k_ab_energy(const cpx<R>* __restrict__ v, int64_t fstride, Geom g,
+                                                   const double* __restrict__ ab, int ieout,
+                                                   double* __restrict__ part) {
+  const int64_t istep = (int64_t)ab[AB_ISTEP];
+  if (ieout <= 0 || istep % ieout != 0) return;
+  double e = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < g.ntot; i += (int64_t)gridDim.x * 256) {
+    const bool x0 = i < g.nmain && i % g.mx == 0;
+    const cpx<R> a = v[i], b = v[i + fstride], c = v[i + 2 * fstride];
+    const double t = (double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
+                     (double)c.x * c.x + (double)c.y * c.y;
+    e += x0 ? 0.5 * t : t;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Constants of the IF-AB2 stepper.
+struct AbConsts {
+  int stepper;      // 1 CFL steps, 2 fixed dt
+  int ieout;
+  double betadx;    // beta pi / lx
+  double dtw;
+  double myeps;     // the output-time tolerance (constant_sp.f90: 2 epsilon(1._sp))
+  double dt_fixed;
+};
+
+// Start of an IF-AB2 step (one thread): the umax refresh, the output time,
+// dt (spectral_dns_dealiased.f90:129-149, 226-232; fixed dt with the half
+// steps of specdns-forced-omp-ft.f90:224-246) and the weights of the update.
+static __global__ void k_ab_begin(double* __restrict__ ab, const double* __restrict__ part, int nparts, AbConsts c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t istep = (int64_t)ab[AB_ISTEP];
+  const double time = ab[AB_TIME], dto = ab[AB_DTO];
+  double dt;
+  if (c.stepper == 2) {
+    dt = istep < 2 ? 0.5 * c.dt_fixed : c.dt_fixed;
+  } else {
+    if (c.ieout > 0 && istep % c.ieout == 0) {
+      double e = 0;
+      for (int b = 0; b < nparts; ++b) e += part[b];
+      ab[AB_UMAX] = 3.5 * sqrt(2.0 * e);
+    }
+    if (c.dtw > 0 && fabs(time - ab[AB_TWRITE]) <= c.myeps) ab[AB_TWRITE] += c.dtw;
+    dt = fmin(c.betadx / ab[AB_UMAX], 3.0 * dto);
+    if (c.dtw > 0) {
+      const double rem = ab[AB_TWRITE] - time;
+      if (dt >= rem) dt = rem;
+      else if (rem - dt < 0.2 * dt) dt = 0.5 * rem;
+    }
+  }
+  double S = 0, ek = 0, del = 0;
+  if (istep == 1) {
+    const double dtt = dt / dto;
+    S = 0.5 * (1.0 + dtt * dtt);
+    ek = 0.5 * (1.0 - dtt * dtt);
+    del = 0.5 * dto * (1.0 + dtt) * (1.0 + dtt);
+  } else if (istep >= 2) {
+    const double dtt = dt / dto;
+    S = (1.0 + 0.5 * dtt) * dt;
+    ek = -0.5 * dtt * dt;
+  }
+  ab[AB_DT] = dt;
+  ab[AB_S] = S;
+  ab[AB_EK] = ek;
+  ab[AB_DELTA] = del;
+}
+
+// End of an IF-AB2 step: time += dt, dto (step 0: dt; step 1: dt + dto;
+// later: dt), the step count.
+static __global__ void k_ab_end(double* __restrict__ ab) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t istep = (int64_t)ab[AB_ISTEP];
+  const double dt = ab[AB_DT];
+  ab[AB_TIME] += dt;
+  ab[AB_DTO] = istep == 1 ? dt + ab[AB_DTO] : dt;
+  ab[AB_ISTEP] = (double)(istep + 1);
 }
 
 // Hermitian copy on the kx = 0 plane (symmetrize.f90:51-61): for kz < 0

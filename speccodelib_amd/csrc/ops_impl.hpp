@@ -207,7 +207,10 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
           case 2: by_zs(IC<2>{}, IC<1>{}); break;
           case 3: by_zs(IC<3>{}, IC<1>{}); break;
           case 4: by_zs(IC<4>{}, IC<1>{}); break;
-          default: set_error("fused z-forward: mode %d is not a stage 1..4", mode); return SCL_EINVAL;
+          case 5: by_zs(IC<5>{}, IC<1>{}); break;
+          case 6: by_zs(IC<6>{}, IC<1>{}); break;
+          case 7: by_zs(IC<7>{}, IC<1>{}); break;
+          default: set_error("fused z-forward: mode %d is not a stage 1..7", mode); return SCL_EINVAL;
         }
         return check_launch("zfwd_zinv");
       }

@@ -63,6 +63,11 @@ struct scl_dns {
   // by the last step's fused stage 4): the next step's stage 1 needs no
   // z-inverse pass.  Cleared whenever v or W change outside a step.
   bool chained = false;
+  // IF-AB2 (p.stepper 1, 2): device state (AB_* slots), energy partials, and
+  // the steps taken since the state was set (selects the bootstrap modes)
+  double* ab = nullptr;
+  double* ab_part = nullptr;
+  int64_t ab_steps = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipEvent_t> events;

@@ -7,6 +7,22 @@ module scl_capi
 
   integer(c_int32_t), parameter :: SCL_SP = 0, SCL_DP = 1
 
+  ! struct scl_dns_params of scl.h (same member order; bind(C) gives the C
+  ! layout and padding)
+  type, bind(C) :: scl_dns_params
+    real(c_double) :: nu = 0, dt = 0
+    integer(c_int32_t) :: forced = 0
+    real(c_double) :: fkmax = 0, eps = 0, eta = 0
+    integer(c_int32_t) :: dealias_rhs = 0, use_graph = 0, sgs = 0
+    real(c_double) :: cs2 = 0, delta = 0, delta_test = 0, k_test = 0
+    integer(c_int32_t) :: iupdate = 10, scalar = 0
+    real(c_double) :: grad = 0, schmidt = 1
+    integer(c_int32_t) :: stepper = 0
+    real(c_double) :: beta = 0.5, dtw = 0
+    integer(c_int32_t) :: ieout = 10
+    real(c_double) :: umax = 5.196152422706632d0, dt_first = 0, kcut = 0
+  end type scl_dns_params
+
   interface
     function scl_last_error() bind(C, name='scl_last_error') result(p)
       import
@@ -101,6 +117,114 @@ module scl_capi
       type(*), dimension(*) :: vx, vy, vz
       real(c_double), value :: nu, dt, fkmax, eps, eta
       integer(c_int32_t), value :: forced
+    end function
+
+    integer(c_int32_t) function scl_smag(plan, vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz, &
+        lx1, ly, lz, nx, ny, nz, delta) bind(C, name='scl_smag')
+      import
+      type(c_ptr), value :: plan
+      type(*), dimension(*) :: vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz
+      integer(c_int32_t), value :: lx1, ly, lz, nx, ny, nz
+      real(c_double), value :: delta
+    end function
+
+    integer(c_int32_t) function scl_dynsmag(plan, ux, uy, uz, t11, t12, t13, t22, t23, t33, kx, ky, kz, g_test, &
+        lx1, ly, lz, nx, ny, nz, lx, delta, delta_test, mm, lm, cs2, update) bind(C, name='scl_dynsmag')
+      import
+      type(c_ptr), value :: plan
+      type(*), dimension(*) :: ux, uy, uz, t11, t12, t13, t22, t23, t33, kx, ky, kz, g_test
+      integer(c_int32_t), value :: lx1, ly, lz, nx, ny, nz, lx, update
+      real(c_double), value :: delta, delta_test
+      real(c_double) :: mm, lm, cs2
+    end function
+
+    integer(c_int32_t) function scl_convecphi_dns(plan, vx, vy, vz, phi, wphi, kx, ky, kz, lx1, ly, lz, &
+        nx, ny, nz) bind(C, name='scl_convecphi_dns')
+      import
+      type(c_ptr), value :: plan
+      type(*), dimension(*) :: vx, vy, vz, phi, wphi, kx, ky, kz
+      integer(c_int32_t), value :: lx1, ly, lz, nx, ny, nz
+    end function
+
+    integer(c_int32_t) function scl_skewness(plan, ux, kx, s, nx, ny, nz) bind(C, name='scl_skewness')
+      import
+      type(c_ptr), value :: plan
+      type(*), dimension(*) :: ux, kx
+      real(c_double) :: s
+      integer(c_int32_t), value :: nx, ny, nz
+    end function
+
+    ! explicit plan handles and the device-resident solver
+    integer(c_int32_t) function scl_plan3d_create(nx, ny, nz, prec, plan) bind(C, name='scl_plan3d_create')
+      import
+      integer(c_int32_t), value :: nx, ny, nz, prec
+      type(c_ptr) :: plan
+    end function
+
+    integer(c_int32_t) function scl_plan3d_destroy(plan) bind(C, name='scl_plan3d_destroy')
+      import
+      type(c_ptr), value :: plan
+    end function
+
+    integer(c_int32_t) function scl_dns_create(plan, params, dns) bind(C, name='scl_dns_create')
+      import
+      type(c_ptr), value :: plan
+      type(scl_dns_params), intent(in) :: params
+      type(c_ptr) :: dns
+    end function
+
+    integer(c_int32_t) function scl_dns_destroy(dns) bind(C, name='scl_dns_destroy')
+      import
+      type(c_ptr), value :: dns
+    end function
+
+    integer(c_int32_t) function scl_dns_set_velocity(dns, vx, vy, vz) bind(C, name='scl_dns_set_velocity')
+      import
+      type(c_ptr), value :: dns
+      type(*), dimension(*) :: vx, vy, vz
+    end function
+
+    integer(c_int32_t) function scl_dns_get_velocity(dns, vx, vy, vz) bind(C, name='scl_dns_get_velocity')
+      import
+      type(c_ptr), value :: dns
+      type(*), dimension(*) :: vx, vy, vz
+    end function
+
+    integer(c_int32_t) function scl_dns_set_scalar(dns, phi) bind(C, name='scl_dns_set_scalar')
+      import
+      type(c_ptr), value :: dns
+      type(*), dimension(*) :: phi
+    end function
+
+    integer(c_int32_t) function scl_dns_get_scalar(dns, phi) bind(C, name='scl_dns_get_scalar')
+      import
+      type(c_ptr), value :: dns
+      type(*), dimension(*) :: phi
+    end function
+
+    integer(c_int32_t) function scl_dns_step(dns, nsteps) bind(C, name='scl_dns_step')
+      import
+      type(c_ptr), value :: dns
+      integer(c_int32_t), value :: nsteps
+    end function
+
+    integer(c_int32_t) function scl_dns_sync(dns) bind(C, name='scl_dns_sync')
+      import
+      type(c_ptr), value :: dns
+    end function
+
+    integer(c_int32_t) function scl_dns_spectrum(dns, ek, nbins) bind(C, name='scl_dns_spectrum')
+      import
+      type(c_ptr), value :: dns
+      real(c_double), dimension(*) :: ek
+      integer(c_int32_t), value :: nbins
+    end function
+
+    ! binds an FP64 solver to the external scl_derivs (derivs(x, y, dydx, ny)
+    ! of rk5.f90:3-10), which a Fortran RK driver then takes as its derivs
+    integer(c_int32_t) function scl_derivs_bind(dns) bind(C, name='scl_derivs_bind')
+      import
+      type(c_ptr), value :: dns
     end function
   end interface
 

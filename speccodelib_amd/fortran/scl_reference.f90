@@ -1,11 +1,13 @@
 ! The reference's module and routine names, argument lists and array
 ! layouts, with bodies that call libscl through module scl_capi.  A program
-! written against the reference (dns-deform.f90 and the drivers of the same
-! family) links these instead of fftwplan.f90, symmetrize.f90,
-! wavenumber.f90, initialize.f90, convec_dns.f90, force.f90,
-! force_heli.f90, dealiasing.f90 and projection.f90, and FFTW 2.x: its
-! rfftwnd_f77_one_complex_to_real / _real_to_complex calls resolve to the
-! library's own entry points of those names.
+! written against the reference (dns-deform.f90, spectral_les.f90,
+! spectral_dns_scalar.f90 and the drivers of the same family) links these
+! instead of fftwplan.f90, symmetrize.f90, wavenumber.f90, initialize.f90,
+! convec_dns.f90, force.f90, force_heli.f90, dealiasing.f90,
+! projection.f90, smag.f90, dynsmag.f90, convecphi_dns.f90 and
+! skewness.f90, and FFTW 2.x: its rfftwnd_f77_one_complex_to_real /
+! _real_to_complex calls resolve to the library's own entry points of those
+! names.
 !
 ! The kind sp comes from the caller's module mconstant (constant_sp.f90 or
 ! constant_dp.f90 in the reference); the plan made by fftwplan3d takes its
@@ -52,7 +54,7 @@ module mwavenumber
   use scl_capi
   implicit none
   interface wavenumber
-    module procedure wavenumberlowmem3d
+    module procedure wavenumberlowmem3d, wavenumberhighmem3d
   end interface wavenumber
 contains
   subroutine wavenumberlowmem3d(kx, ky, kz, k2, lx1, ly, lz)
@@ -63,6 +65,23 @@ contains
     real(sp), dimension(lx1, ly, lz), intent(out) :: k2
     call scl_check(scl_wavenumber3d(plan_prec(), kx, ky, kz, k2, lx1, ly, lz), 'wavenumber')
   end subroutine wavenumberlowmem3d
+
+  ! wavenumber.f90:131-158: the same wavenumbers as 3D arrays (the LES
+  ! drivers' form, spectral_les.f90:91,99)
+  subroutine wavenumberhighmem3d(kx, ky, kz, k2, lx1, ly, lz)
+    integer, intent(in) :: lx1, ly, lz
+    real(sp), dimension(lx1, ly, lz), intent(out) :: kx, ky, kz, k2
+    real(sp) :: ax(lx1), ay(ly), az(lz)
+    integer :: jj, kk
+    call scl_check(scl_wavenumber3d(plan_prec(), ax, ay, az, k2, lx1, ly, lz), 'wavenumber')
+    do kk = 1, lz
+      do jj = 1, ly
+        kx(:, jj, kk) = ax
+        ky(:, jj, kk) = ay(jj)
+        kz(:, jj, kk) = az(kk)
+      end do
+    end do
+  end subroutine wavenumberhighmem3d
 end module mwavenumber
 
 ! symmetrize.f90:1-6, 32-64 (module msymmetrize, the 3D form)
@@ -202,3 +221,84 @@ subroutine rk4step(vx, vy, vz, lx1, ly, lz, rnu, dt, iforce, fkmax, eps, eta)
   call scl_check(scl_rk4step(c_null_ptr, vx, vy, vz, real(rnu, c_double), real(dt, c_double), iforce, &
                              real(fkmax, c_double), real(eps, c_double), real(eta, c_double)), 'rk4step')
 end subroutine rk4step
+
+! smag.f90:1-78 (c1 = 0.026 inside the library)
+subroutine smag(vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz, lx1, ly, lz, nx, ny, nz, delta)
+  use iso_c_binding
+  use mconstant
+  use scl_capi
+  implicit none
+  integer, intent(in) :: lx1, ly, lz, nx, ny, nz
+  real(sp), intent(in) :: delta
+  complex(sp), dimension(lx1, ly, lz), intent(in) :: vx, vy, vz
+  complex(sp), dimension(lx1, ly, lz), intent(out) :: t11, t12, t13, t22, t23, t33
+  real(sp), dimension(lx1), intent(in) :: kx
+  real(sp), dimension(ly), intent(in) :: ky
+  real(sp), dimension(lz), intent(in) :: kz
+  call scl_check(scl_smag(c_null_ptr, vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz, lx1, ly, lz, &
+                          nx, ny, nz, real(delta, c_double)), 'smag')
+end subroutine smag
+
+! dynsmag.f90:1-236.  The logical update and the real(sp) mm, lm, cs2 of the
+! reference become the C ABI's int32 and doubles here; with update false
+! cs2 is read (the static branch, :196-233) and mm, lm are left as they are.
+subroutine dynsmag(ux, uy, uz, t11, t12, t13, t22, t23, t33, kx, ky, kz, g_test, &
+                   lx1, ly, lz, nx, ny, nz, lx, delta, delta_test, mm, lm, cs2, update)
+  use iso_c_binding
+  use mconstant
+  use scl_capi
+  implicit none
+  integer, intent(in) :: lx1, ly, lz, nx, ny, nz, lx
+  real(sp), intent(in) :: delta, delta_test
+  logical, intent(in) :: update
+  real(sp), intent(inout) :: cs2, mm, lm
+  complex(sp), dimension(lx1, ly, lz), intent(in) :: ux, uy, uz
+  real(sp), dimension(lx1, ly, lz), intent(in) :: kx, ky, kz, g_test
+  complex(sp), dimension(lx1, ly, lz), intent(out) :: t11, t12, t13, t22, t23, t33
+  real(c_double) :: dmm, dlm, dcs2
+  integer(c_int32_t) :: iu
+  dmm = mm
+  dlm = lm
+  dcs2 = cs2
+  iu = 0
+  if (update) iu = 1
+  call scl_check(scl_dynsmag(c_null_ptr, ux, uy, uz, t11, t12, t13, t22, t23, t33, kx, ky, kz, g_test, &
+                             lx1, ly, lz, nx, ny, nz, lx, real(delta, c_double), real(delta_test, c_double), &
+                             dmm, dlm, dcs2, iu), 'dynsmag')
+  if (update) then
+    mm = real(dmm, sp)
+    lm = real(dlm, sp)
+    cs2 = real(dcs2, sp)
+  end if
+end subroutine dynsmag
+
+! convecphi_dns.f90:1-49: wphi = i k . FFT(u phi)/N^3 (the reference's sign)
+subroutine convecphi_dns(vx, vy, vz, phi, wphi, kx, ky, kz, lx1, ly, lz, nx, ny, nz)
+  use iso_c_binding
+  use mconstant
+  use scl_capi
+  implicit none
+  integer, intent(in) :: lx1, ly, lz, nx, ny, nz
+  complex(sp), dimension(lx1, ly, lz), intent(in) :: vx, vy, vz, phi
+  complex(sp), dimension(lx1, ly, lz), intent(inout) :: wphi
+  real(sp), dimension(lx1), intent(in) :: kx
+  real(sp), dimension(ly), intent(in) :: ky
+  real(sp), dimension(lz), intent(in) :: kz
+  call scl_check(scl_convecphi_dns(c_null_ptr, vx, vy, vz, phi, wphi, kx, ky, kz, lx1, ly, lz, nx, ny, nz), &
+                 'convecphi_dns')
+end subroutine convecphi_dns
+
+! skewness.f90:1-48
+subroutine skewness(ux, kx, s, nx, ny, nz)
+  use iso_c_binding
+  use mconstant
+  use scl_capi
+  implicit none
+  integer, intent(in) :: nx, ny, nz
+  complex(sp), dimension(nx / 2 + 1, ny, nz), intent(in) :: ux
+  real(sp), dimension(nx / 2 + 1), intent(in) :: kx
+  real(sp), intent(out) :: s
+  real(c_double) :: ds
+  call scl_check(scl_skewness(c_null_ptr, ux, kx, ds, nx, ny, nz), 'skewness')
+  s = real(ds, sp)
+end subroutine skewness
