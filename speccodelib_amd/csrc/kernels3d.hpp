@@ -266,9 +266,9 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zinv_vort(const
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const uint32_t o = zo.at(rt + T * m);
-    uz[m] = s[o + 2 * sstride];
-    uy[m] = s[o + sstride];
-    ux[m] = s[o];
+    uz[m] = ld_once<R, (CW >= 16)>(s + o + 2 * sstride);
+    uy[m] = ld_once<R, (CW >= 16)>(s + o + sstride);
+    ux[m] = ld_once<R, (CW >= 16)>(s + o);
   }
   zinv_store<R, N, E, CW, PARK, ZS>(ux, uy, uz, lds, park, twl, rt, c, (R)t.kx, (R)kwave(t.oth, g.ny), g.nz, zo, w,
                                     wstride);
@@ -319,19 +319,20 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
 // or the new v (MODE 4, always stored).
 // MODE 4 with nf set: a non-finite new v raises the sticky flag *nf (read
 // first, so that a field full of NaN does not make every lane an atomic).
-template <int MODE, class R, class I>
+// NT: nontemporal loads of v and du (whole-line accesses only, ld_once).
+template <int MODE, class R, class I, bool NT = false>
 __device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
                                     bool store_s, unsigned* nf = nullptr) {
   cpx<R> out;
   if constexpr (MODE == 1) {
-    const cpx<R> v = V[o];
+    const cpx<R> v = ld_once<R, NT>(V + o);
     const cpx<R> u = v + ((R)1 / (R)6) * a, q = v + (R)0.5 * a;
     DU[o] = u + expm1_r((R)2 * x) * u;
     out = q + em1 * q;
   } else if constexpr (MODE == 2 || MODE == 3) {
-    const cpx<R> v = V[o];
+    const cpx<R> v = ld_once<R, NT>(V + o);
     const cpx<R> q = ((R)1 / (R)3) * a;
-    DU[o] = DU[o] + (q + em1 * q);
+    DU[o] = ld_once<R, NT>(DU + o) + (q + em1 * q);
     const cpx<R> ev = v + em1 * v;
     if constexpr (MODE == 2) {
       out = ev + (R)0.5 * a;
@@ -340,7 +341,7 @@ __device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>
       out = u + em1 * u;
     }
   } else {
-    out = cut ? mk<R>(0, 0) : DU[o] + ((R)1 / (R)6) * a;
+    out = cut ? mk<R>(0, 0) : ld_once<R, NT>(DU + o) + ((R)1 / (R)6) * a;
     V[o] = out;
     if (nf && !(isfinite(out.x) && isfinite(out.y)) && *(volatile unsigned*)nf == 0u) atomicOr(nf, 1u);
     return out;
@@ -381,20 +382,20 @@ enum { AB_TIME = 0, AB_DT = 1, AB_DTO = 2, AB_UMAX = 3, AB_TWRITE = 4, AB_ISTEP 
 //   MODE 6 (step 1):  v2 = [cS E(dto) v0 + cek v1 + cdel a] E(dt)  (v0 from DU)
 //   MODE 7 (>= 2):    v = [v + cS a + cek E(dto) O] E(dt);  S = a (O = S)
 // The new v is 0 where cut (k2 >= kcut^2, and the mean).  nf: as stage_keep.
-template <int MODE, class R, class I>
+template <int MODE, class R, class I, bool NT = false>
 __device__ inline cpx<R> ab_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS, R cek, R cdel, R e1,
                                  R e2, bool cut, unsigned* nf) {
-  const cpx<R> v = V[o];
+  const cpx<R> v = ld_once<R, NT>(V + o);
   cpx<R> y;
   if constexpr (MODE == 5) {
     DU[o] = v;
     S[o] = a;
     y = v + dt * a;
   } else if constexpr (MODE == 6) {
-    const cpx<R> v0 = DU[o];
+    const cpx<R> v0 = ld_once<R, NT>(DU + o);
     y = cS * (v0 + e2 * v0) + cek * v + cdel * a;
   } else {
-    const cpx<R> ob = S[o];
+    const cpx<R> ob = ld_once<R, NT>(S + o);
     y = v + cS * a + cek * (ob + e2 * ob);
     S[o] = a;
   }
@@ -452,9 +453,9 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const uint32_t o = zo.at(rt + T * m);
-    nx_[m] = W[o];
-    ny_[m] = W1[o];
-    if constexpr (!PARK) nz_[m] = W2[o];
+    nx_[m] = ld_once<R, (CW >= 16)>(W + o);
+    ny_[m] = ld_once<R, (CW >= 16)>(W1 + o);
+    if constexpr (!PARK) nz_[m] = ld_once<R, (CW >= 16)>(W2 + o);
   }
   using L = Line<R, N, E, -1>;
   L::run(nx_, lds, rt, twl, 1, ColIdx<CW>{c});
@@ -462,7 +463,7 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       park[m * CC::BS + threadIdx.x] = nx_[m];
-      nz_[m] = W2[zo.at(rt + T * m)];
+      nz_[m] = ld_once<R, (CW >= 16)>(W2 + zo.at(rt + T * m));
     }
   }
   L::run(ny_, lds, rt, twl, 1, ColIdx<CW>{c});
@@ -537,11 +538,12 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R e1 = expm1_r(xk * ab_dt);
       const R e2 = MODE == 5 ? (R)0 : expm1_r(xk * ab_dto);
       const bool cut = k2 >= p.kc2 || ik2 == 0;
-      nx_[m] = ab_keep<MODE, R>(a, o, V, DU, S, ab_dt, ab_s, ab_ek, ab_del, e1, e2, cut, p.nonfinite);
-      ny_[m] = ab_keep<MODE, R>(b, o, V + fstride, DU + fstride, S + fstride, ab_dt, ab_s, ab_ek, ab_del, e1, e2,
-                                cut, p.nonfinite);
-      nz_[m] = ab_keep<MODE, R>(d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, ab_dt, ab_s, ab_ek, ab_del,
-                                e1, e2, cut, p.nonfinite);
+      constexpr bool NT = CW >= 16;
+      nx_[m] = ab_keep<MODE, R, uint32_t, NT>(a, o, V, DU, S, ab_dt, ab_s, ab_ek, ab_del, e1, e2, cut, p.nonfinite);
+      ny_[m] = ab_keep<MODE, R, uint32_t, NT>(b, o, V + fstride, DU + fstride, S + fstride, ab_dt, ab_s, ab_ek, ab_del,
+                                              e1, e2, cut, p.nonfinite);
+      nz_[m] = ab_keep<MODE, R, uint32_t, NT>(d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, ab_dt, ab_s,
+                                              ab_ek, ab_del, e1, e2, cut, p.nonfinite);
     } else {
       // E = exp(x), x = -nu k2 dt/2.  E is within 1e-3 of 1 for most modes,
       // where FP32 spacing (6e-8) makes a rounded E or E*E a biased factor
@@ -551,11 +553,12 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R em1 = expm1_r(x);
       const bool cut = k2 >= p.kc2;
       const bool store_s = !FUSED || (p.forced && k2 < p.fk2);
-      nx_[m] = stage_keep<MODE, R>(p.dt * a, o, V, DU, S, em1, x, cut, store_s, p.nonfinite);
-      ny_[m] = stage_keep<MODE, R>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut, store_s,
-                                   p.nonfinite);
-      nz_[m] = stage_keep<MODE, R>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1, x, cut,
-                                   store_s, p.nonfinite);
+      constexpr bool NT = CW >= 16;
+      nx_[m] = stage_keep<MODE, R, uint32_t, NT>(p.dt * a, o, V, DU, S, em1, x, cut, store_s, p.nonfinite);
+      ny_[m] = stage_keep<MODE, R, uint32_t, NT>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut,
+                                                 store_s, p.nonfinite);
+      nz_[m] = stage_keep<MODE, R, uint32_t, NT>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1,
+                                                 x, cut, store_s, p.nonfinite);
     }
   }
   if constexpr (FUSED && MODE > 0) {
