@@ -392,6 +392,31 @@ int32_t scl_adjoint_vjp(scl_adjoint *adj, double cs2, const void *sx, const void
 int32_t scl_adjoint_timing(scl_adjoint *adj, double *forward_ms, double *backward_ms);
 
 /* ---------------------------------------------------------------------
+ * Files of the reference's runs (host side).
+ * ------------------------------------------------------------------- */
+/* One Fortran unformatted sequential record (4-byte length markers; records
+ * over 2^31 - 9 bytes split into subrecords as gfortran writes them), as
+ * output.f90:103-166 writes a field with write(10) and input.f90:106-170
+ * reads it.  scl_record_read requires the file to be exactly one record of
+ * nbytes: a short, truncated or corrupted file returns SCL_EINVAL. */
+int32_t scl_record_write(const char *path, const void *data, int64_t nbytes);
+int32_t scl_record_read(const char *path, void *data, int64_t nbytes);
+/* output(vx,vy,vz[,phi],idump,...) (output.f90:103-166): the solver's
+ * fields to <dir>/ux<idump>.dat, uy, uz (and phi with a scalar), one record
+ * each in the reference layout; dir NULL = ".".  scl_dns_input
+ * (input.f90:106-170) reads them back (every file checked before the state
+ * is set); a dump written after step k and read into a solver of the same
+ * configuration continues the IF-RK4 run bit-identically. */
+int32_t scl_dns_output(scl_dns *d, const char *dir, int32_t idump);
+int32_t scl_dns_input(scl_dns *d, const char *dir, int32_t idump);
+/* Appends one line of ./post/ener_time.data: write(25,'(I6,20E12.4)')
+ * istep, vals(1:nvals) (spectral_dns_dealiased.f90:136, nvals <= 20). */
+int32_t scl_write_ener_time(const char *path, int32_t istep, const double *vals, int32_t nvals);
+/* Appends one block of ./post/spectrum.data: the time, then "ll E(ll)" for
+ * ll = 1..n, list-directed (spectral_dns_dealiased.f90:158-171). */
+int32_t scl_write_spectrum(const char *path, double time, const double *ek, int32_t n);
+
+/* ---------------------------------------------------------------------
  * Measurement.  No counterpart in the reference: the HBM rate a plain
  * streaming kernel reaches on the current device (16-byte vector accesses,
  * grid-stride, buffers of `bytes` each), the practical ceiling next to the

@@ -187,6 +187,12 @@ def lib():
             "scl_adjoint_vjp": ([P, D] + [P] * 9 + [P, I32], I32),
             "scl_adjoint_timing": ([P, P, P], I32),
             "scl_stream_probe": ([I32, ctypes.c_int64, I32, P], I32),
+            "scl_record_write": ([ctypes.c_char_p, P, ctypes.c_int64], I32),
+            "scl_record_read": ([ctypes.c_char_p, P, ctypes.c_int64], I32),
+            "scl_dns_output": ([P, ctypes.c_char_p, I32], I32),
+            "scl_dns_input": ([P, ctypes.c_char_p, I32], I32),
+            "scl_write_ener_time": ([ctypes.c_char_p, I32, P, I32], I32),
+            "scl_write_spectrum": ([ctypes.c_char_p, D, P, I32], I32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -465,6 +471,15 @@ class DNS:
     def rates(self, rate_ptr):
         _check(lib().scl_dns_rates(self.h, rate_ptr))
 
+    def output(self, directory=".", idump=0):
+        """output.f90:103-166: <dir>/ux<idump>.dat, uy, uz [, phi] as Fortran
+        unformatted sequential records."""
+        _check(lib().scl_dns_output(self.h, os.fsencode(directory), idump))
+
+    def input(self, directory=".", idump=0):
+        """input.f90:106-170: sets the state from <dir>/ux<idump>.dat, ..."""
+        _check(lib().scl_dns_input(self.h, os.fsencode(directory), idump))
+
     def step_timed(self, n):
         """Runs n steps with HIP events around every kernel launch; returns
         (total_ms, {kind: (ms, launches)})."""
@@ -627,6 +642,31 @@ class Adjoint:
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().scl_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+
+def record_write(path, a):
+    """One Fortran unformatted sequential record holding the array a."""
+    a = np.ascontiguousarray(a)
+    _check(lib().scl_record_write(os.fsencode(path), a.ctypes.data, a.nbytes))
+
+
+def record_read(path, shape, dtype):
+    """The array of one Fortran unformatted sequential record (exactly)."""
+    out = np.zeros(shape, dtype)
+    _check(lib().scl_record_read(os.fsencode(path), out.ctypes.data, out.nbytes))
+    return out
+
+
+def write_ener_time(path, istep, *vals):
+    """Appends write(25,'(I6,20E12.4)') istep, vals (the ener_time.data line)."""
+    v = np.array(vals, np.float64)
+    _check(lib().scl_write_ener_time(os.fsencode(path), istep, v.ctypes.data, len(v)))
+
+
+def write_spectrum(path, time, ek):
+    """Appends the time and the "ll E(ll)" lines of spectrum.data."""
+    e = np.ascontiguousarray(ek, np.float64)
+    _check(lib().scl_write_spectrum(os.fsencode(path), time, e.ctypes.data, len(e)))
 
 
 def stream_probe(kind, nbytes, reps=10):

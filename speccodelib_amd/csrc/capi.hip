@@ -571,13 +571,15 @@ static int32_t rk_stage(scl_plan3d* p, const DevFields& d, int mode, const RhsPa
   tm->mark(fuse ? (mode >= 4 ? SCL_K_ZFWD4_ZINV : SCL_K_ZFWD_ZINV) : SCL_K_ZFWD);
   SCL_CHECK(Ops<R>::zfwd_rhs(*p, mode, W, fs, V, DU, S, st, d.coef, rp, fuse));
   // The kx = 0 Hermitian copy (symmetrize.f90:51-61) goes on N (mode 0),
-  // on the collected stage states (s_out: the adjoint's tape), and on v once
-  // per step.  Skipping it on du and the stage state of stages 1..3 changes
-  // them only at rounding level: the kx = 0 plane of N is Hermitian up to
-  // rounding (it is the transform of real products), and c2r ignores the
-  // anti-Hermitian part of the kx = 0 lines.  For the same reason the fused
-  // stage 4 may z-inverse v before its kx = 0 copy.
-  if (mode >= 1 && mode < 4 && !d.s_out) {
+  // on the collected stage states (s_out: the adjoint's tape), and on v at
+  // the end of an unchained step.  Skipping it on du and the stage states of
+  // stages 1..3, and on v when the last stage z-inverses it for the next
+  // step (fuse), changes them only at rounding level: the kx = 0 plane of N
+  // is Hermitian up to rounding (it is the transform of real products), and
+  // c2r ignores the anti-Hermitian part of the kx = 0 lines.  Leaving v as
+  // the chain transformed it also makes a dump of v the exact state the
+  // chain continues from, so a restart reproduces the run bit for bit.
+  if (((mode >= 1 && mode < 4) || (mode >= 4 && fuse)) && !d.s_out) {
     tm->mark(-1);
     return SCL_OK;
   }
