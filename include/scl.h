@@ -139,6 +139,35 @@ int32_t scl_convecphi_dns(scl_plan3d *plan, const void *vx, const void *vy, cons
                           void *wphi, const void *kx, const void *ky, const void *kz, int32_t lx1, int32_t ly,
                           int32_t lz, int32_t nx, int32_t ny, int32_t nz);
 
+/* 3/2-rule padded products.  padd(u,ub,lx1,ly,lz,lxb1,lyb,lzb)
+ * (padd.f90:1-14) and unpadd(ub,u,...) (unpadd.f90:1-13): host-array index
+ * copies between a (lx1,ly,lz) half spectrum and a zero-padded
+ * (lxb1,lyb,lzb) one, any sizes, in the real type of prec. */
+int32_t scl_padd(int32_t prec, const void *u, void *ub, int32_t lx1, int32_t ly, int32_t lz, int32_t lxb1,
+                 int32_t lyb, int32_t lzb);
+int32_t scl_unpadd(int32_t prec, const void *ub, void *u, int32_t lx1, int32_t ly, int32_t lz, int32_t lxb1,
+                   int32_t lyb, int32_t lzb);
+/* The padded convec(vx,vy,vz,wx,wy,wz,lx1,ly,lz,lxb1,lyb,lzb,nxb,nyb,nzb)
+ * of spectral_heli_dynsmag.f90:229 (plans of fftwplan_les.f90:11-21):
+ * w <- unpadd(FFT_b(u x w))/(nxb nyb nzb), u, w from the padded inverse
+ * transforms.  The padded sizes must be at least 3/2 of the plan's; the
+ * library pads to the next power of two per axis (2 n), on which the
+ * product is alias-free on every mode unpadd keeps, as on the 3n/2 grid. */
+int32_t scl_convec_padded(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, void *wx, void *wy,
+                          void *wz, int32_t lx1, int32_t ly, int32_t lz, int32_t lxb1, int32_t lyb, int32_t lzb,
+                          int32_t nxb, int32_t nyb, int32_t nzb);
+/* smag(vx,vy,vz,tau11,...,tau33,kx,ky,kz,lx1,ly,lz,lxb1,lyb,lzb,nxb,nyb,nzb,delta)
+ * of smag_dealias.f90:1-78 (c1 = 0.026, kx/ky/kz 3D arrays), with that
+ * file's strain (off-diagonal terms without the 1/2): tau_ij =
+ * unpadd(FFT_b(-2 c1 delta^2 |S| S_ij))/(padded points), on the padded grid
+ * of scl_convec_padded.  |S| S_ij is not a polynomial, so its aliasing on
+ * that grid differs from the 3n/2 grid's at the level of the field's
+ * spectrum near the padded cutoff. */
+int32_t scl_smag_dealias(scl_plan3d *plan, const void *vx, const void *vy, const void *vz, void *t11, void *t12,
+                         void *t13, void *t22, void *t23, void *t33, const void *kx, const void *ky, const void *kz,
+                         int32_t lx1, int32_t ly, int32_t lz, int32_t lxb1, int32_t lyb, int32_t lzb, int32_t nxb,
+                         int32_t nyb, int32_t nzb, double delta);
+
 /* skewness(ux,kx,s,nx,ny,nz) (skewness.f90:1-48): s = <d^3>/<d^2>^(3/2),
  * d = du_x/dx in physical space, summed in FP64.  kx must be wavenumber()'s. */
 int32_t scl_skewness(scl_plan3d *plan, const void *ux, const void *kx, double *s, int32_t nx, int32_t ny, int32_t nz);

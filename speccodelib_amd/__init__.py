@@ -159,6 +159,10 @@ def lib():
             "scl_dynsmag": ([P] + [P] * 13 + [I32] * 7 + [D, D, P, P, P, I32], I32),
             "scl_convecphi_dns": ([P] + [P] * 8 + [I32] * 6, I32),
             "scl_skewness": ([P, P, P, P, I32, I32, I32], I32),
+            "scl_padd": ([I32, P, P] + [I32] * 6, I32),
+            "scl_unpadd": ([I32, P, P] + [I32] * 6, I32),
+            "scl_convec_padded": ([P] + [P] * 6 + [I32] * 9, I32),
+            "scl_smag_dealias": ([P] + [P] * 12 + [I32] * 9 + [D], I32),
             "scl_ens2d_create": ([I32, I32, P, P, ctypes.POINTER(ScEns2DParams), ctypes.POINTER(P)], I32),
             "scl_ens2d_destroy": ([P], I32),
             "scl_ens2d_set_member": ([P, I32, I32, P], I32),
@@ -329,6 +333,27 @@ class Plan3D:
         _check(lib().scl_convecphi_dns(self.h, *map(_ptr, (vx, vy, vz, phi, w, kx, ky, kz)), *self._dims(),
                                        self.nx, self.ny, self.nz))
         return w
+
+    def padded_shape(self):
+        """The reference's 3/2-rule padded half-spectrum shape (lzb, lyb, lxb1)."""
+        return (3 * self.nz // 2, 3 * self.ny // 2, 3 * self.nx // 4 + 1)
+
+    def convec_padded(self, vx, vy, vz, wx, wy, wz):
+        """The padded convec of spectral_heli_dynsmag.f90:229 (3/2 rule):
+        w <- the unaliased Lamb vector, in place."""
+        for a in (vx, vy, vz, wx, wy, wz):
+            _field(a, self.prec, self.shape)
+        nxb, nyb, nzb = 3 * self.nx // 2, 3 * self.ny // 2, 3 * self.nz // 2
+        _check(lib().scl_convec_padded(self.h, *map(_ptr, (vx, vy, vz, wx, wy, wz)), *self._dims(),
+                                       nxb // 2 + 1, nyb, nzb, nxb, nyb, nzb))
+
+    def smag_dealias(self, vx, vy, vz, kx3, ky3, kz3, delta):
+        """smag of smag_dealias.f90 (3/2-padded products): (t11 .. t33)."""
+        t = [self.zeros() for _ in range(6)]
+        nxb, nyb, nzb = 3 * self.nx // 2, 3 * self.ny // 2, 3 * self.nz // 2
+        _check(lib().scl_smag_dealias(self.h, *map(_ptr, (vx, vy, vz, *t, kx3, ky3, kz3)), *self._dims(),
+                                      nxb // 2 + 1, nyb, nzb, nxb, nyb, nzb, delta))
+        return tuple(t)
 
     def skewness(self, ux, kx):
         """skewness.f90:1-48: <(du_x/dx)^3> / <(du_x/dx)^2>^(3/2), FP64."""
@@ -642,6 +667,26 @@ class Adjoint:
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().scl_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+
+def padd(u, shape_b):
+    """padd.f90: u (lz, ly, lx1) zero-padded into shape_b (lzb, lyb, lxb1)."""
+    prec = SP if u.dtype == np.complex64 else DP
+    ub = np.zeros(shape_b, u.dtype)
+    lz, ly, lx1 = u.shape
+    _check(lib().scl_padd(prec, _ptr(np.ascontiguousarray(u)), _ptr(ub), lx1, ly, lz, shape_b[2], shape_b[1],
+                          shape_b[0]))
+    return ub
+
+
+def unpadd(ub, shape):
+    """unpadd.f90: the (lz, ly, lx1) part of a padded half spectrum."""
+    prec = SP if ub.dtype == np.complex64 else DP
+    u = np.zeros(shape, ub.dtype)
+    lzb, lyb, lxb1 = ub.shape
+    _check(lib().scl_unpadd(prec, _ptr(np.ascontiguousarray(ub)), _ptr(u), shape[2], shape[1], shape[0], lxb1, lyb,
+                            lzb))
+    return u
 
 
 def record_write(path, a):

@@ -289,6 +289,7 @@ extern "C" int32_t scl_plan3d_destroy(scl_plan3d* p) {
   if (p->coef) (void)hipFree(p->coef);
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->diag) (void)hipFree(p->diag);
+  if (p->padded) scl_plan3d_destroy(p->padded);
   if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return SCL_OK;
@@ -1439,6 +1440,166 @@ extern "C" int32_t scl_convecphi_dns(scl_plan3d* p, const void* vx, const void* 
     hipLaunchKernelGGL((k_divflux<R>), ExtOps<R>::grid_all(g), dim3(256), 0, p->stream, W, fs, g);
     SCL_CHECK(check_launch("divflux"));
     return download<R>(p, W, wphi);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// 3/2-rule padded products (padd.f90, unpadd.f90, fftwplan_les.f90,
+// smag_dealias.f90).  The padded grid here is the smallest power of two per
+// axis that is at least 3n/2 (2n for the plan's powers of two): a product of
+// two fields band-limited to |k_i| < n/2 is alias-free on every grid of at
+// least 3n/2 points per axis, so the modes kept by unpadd come out as on the
+// reference's 3n/2 grid, while the transforms stay radix-2/4/8/16.
+static dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+static int pad_size(int n) {
+  int m = 1;
+  while (m < 3 * n / 2) m <<= 1;
+  return m;
+}
+
+static int32_t padded_plan(scl_plan3d* p, scl_plan3d** out) {
+  if (!p->padded) {
+    scl_plan3d* b = nullptr;
+    SCL_CHECK(scl_plan3d_create(pad_size(p->g.nx), pad_size(p->g.ny), pad_size(p->g.nz), p->prec, &b));
+    const int32_t st = scl_plan3d_set_stream(b, p->stream);
+    if (st != SCL_OK) {
+      scl_plan3d_destroy(b);
+      return st;
+    }
+    p->padded = b;
+  } else if (p->padded->stream != p->stream) {
+    SCL_CHECK(scl_plan3d_set_stream(p->padded, p->stream));
+  }
+  *out = p->padded;
+  return SCL_OK;
+}
+
+static int32_t check_pad_dims(const scl_plan3d* p, int lxb1, int lyb, int lzb, int nxb, int nyb, int nzb) {
+  const Geom& g = p->g;
+  if (lxb1 != nxb / 2 + 1 || lyb != nyb || lzb != nzb || 2 * nxb < 3 * g.nx || 2 * nyb < 3 * g.ny ||
+      2 * nzb < 3 * g.nz)
+    return set_error("padded sizes (%d,%d,%d) / (%d,%d,%d): need lxb1 = nxb/2+1, lyb = nyb, lzb = nzb, each at "
+                     "least 3/2 of the plan's grid",
+                     lxb1, lyb, lzb, nxb, nyb, nzb),
+           SCL_EINVAL;
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_padd(int32_t prec, const void* u, void* ub, int32_t lx1, int32_t ly, int32_t lz, int32_t lxb1,
+                            int32_t lyb, int32_t lzb) {
+  if (!u || !ub || lx1 < 1 || ly < 2 || lz < 2 || lxb1 < lx1 || lyb < ly || lzb < lz || (ly & 1) || (lz & 1))
+    return set_error("bad padd arguments"), SCL_EINVAL;
+  const size_t es = prec == SCL_SP ? 8 : 16;
+  const char* a = (const char*)u;
+  char* b = (char*)ub;
+  std::memset(b, 0, (size_t)lxb1 * lyb * lzb * es);
+  for (int z = 0; z < lz; ++z) {
+    const int zb = z < lz / 2 ? z : lzb - lz + z;
+    for (int y = 0; y < ly; ++y) {
+      const int yb = y < ly / 2 ? y : lyb - ly + y;
+      std::memcpy(b + (((size_t)zb * lyb + yb) * lxb1) * es, a + (((size_t)z * ly + y) * lx1) * es, (size_t)lx1 * es);
+    }
+  }
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_unpadd(int32_t prec, const void* ub, void* u, int32_t lx1, int32_t ly, int32_t lz,
+                              int32_t lxb1, int32_t lyb, int32_t lzb) {
+  if (!u || !ub || lx1 < 1 || ly < 2 || lz < 2 || lxb1 < lx1 || lyb < ly || lzb < lz || (ly & 1) || (lz & 1))
+    return set_error("bad unpadd arguments"), SCL_EINVAL;
+  const size_t es = prec == SCL_SP ? 8 : 16;
+  const char* b = (const char*)ub;
+  char* a = (char*)u;
+  for (int z = 0; z < lz; ++z) {
+    const int zb = z < lz / 2 ? z : lzb - lz + z;
+    for (int y = 0; y < ly; ++y) {
+      const int yb = y < ly / 2 ? y : lyb - ly + y;
+      std::memcpy(a + (((size_t)z * ly + y) * lx1) * es, b + (((size_t)zb * lyb + yb) * lxb1) * es, (size_t)lx1 * es);
+    }
+  }
+  return SCL_OK;
+}
+
+extern "C" int32_t scl_convec_padded(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* wx,
+                                     void* wy, void* wz, int32_t lx1, int32_t ly, int32_t lz, int32_t lxb1,
+                                     int32_t lyb, int32_t lzb, int32_t nxb, int32_t nyb, int32_t nzb) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  SCL_CHECK(check_pad_dims(p, lxb1, lyb, lzb, nxb, nyb, nzb));
+  scl_plan3d* pb;
+  SCL_CHECK(padded_plan(p, &pb));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    void *s, *sb;
+    SCL_CHECK(plan_scratch(p, 1, &s));
+    SCL_CHECK(plan_scratch(pb, 6, &sb));
+    cpx<R>* U = (cpx<R>*)s;
+    cpx<R>* B = (cpx<R>*)sb;
+    const int64_t fsb = pb->fstride();
+    const void* in[6] = {vx, vy, vz, wx, wy, wz};
+    for (int f = 0; f < 6; ++f) {
+      SCL_CHECK(upload<R>(p, in[f], U));
+      hipLaunchKernelGGL((k_pad<R>), grid1(pb->g.ntot), dim3(256), 0, p->stream, U, p->g, B + f * fsb, pb->g);
+      SCL_CHECK(check_launch("pad"));
+    }
+    SCL_CHECK(Ops<R>::colfft(*pb, B, fsb, 6, 2, +1));
+    SCL_CHECK(Ops<R>::colfft(*pb, B, fsb, 6, 1, +1));
+    SCL_CHECK(Ops<R>::row_convec(*pb, B, fsb));  // u x w / (Mx My Mz)
+    SCL_CHECK(Ops<R>::colfft(*pb, B, fsb, 3, 1, -1));
+    SCL_CHECK(Ops<R>::colfft(*pb, B, fsb, 3, 2, -1));
+    void* out[3] = {wx, wy, wz};
+    for (int f = 0; f < 3; ++f) {
+      hipLaunchKernelGGL((k_unpad<R>), grid1(p->g.ntot), dim3(256), 0, p->stream, B + f * fsb, pb->g, U, p->g);
+      SCL_CHECK(check_launch("unpad"));
+      SCL_CHECK(download<R>(p, U, out[f]));
+    }
+    return SCL_OK;
+  });
+}
+
+extern "C" int32_t scl_smag_dealias(scl_plan3d* p, const void* vx, const void* vy, const void* vz, void* t11,
+                                    void* t12, void* t13, void* t22, void* t23, void* t33, const void* kx,
+                                    const void* ky, const void* kz, int32_t lx1, int32_t ly, int32_t lz,
+                                    int32_t lxb1, int32_t lyb, int32_t lzb, int32_t nxb, int32_t nyb, int32_t nzb,
+                                    double delta) {
+  SCL_PLAN(p);
+  SCL_CHECK(check_dims(p, lx1, ly, lz));
+  SCL_CHECK(check_pad_dims(p, lxb1, lyb, lzb, nxb, nyb, nzb));
+  scl_plan3d* pb;
+  SCL_CHECK(padded_plan(p, &pb));
+  return by_prec(p->prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    SCL_CHECK(check_k3d<R>(p, kx, ky, kz));
+    void *s, *sb;
+    SCL_CHECK(plan_scratch(p, 9, &s));
+    SCL_CHECK(plan_scratch(pb, 6, &sb));
+    cpx<R>* V = (cpx<R>*)s;
+    const int64_t fs = p->fstride(), fsb = pb->fstride();
+    cpx<R>* S = V + 3 * fs;
+    cpx<R>* B = (cpx<R>*)sb;
+    const void* in[3] = {vx, vy, vz};
+    for (int f = 0; f < 3; ++f) SCL_CHECK(upload<R>(p, in[f], V + f * fs));
+    hipLaunchKernelGGL((k_strain_smagd<R>), grid1(p->g.ntot), dim3(256), 0, p->stream, V, S, fs, p->g);
+    SCL_CHECK(check_launch("strain_smagd"));
+    for (int q = 0; q < 6; ++q) {
+      hipLaunchKernelGGL((k_pad<R>), grid1(pb->g.ntot), dim3(256), 0, p->stream, S + q * fs, p->g, B + q * fsb, pb->g);
+      SCL_CHECK(check_launch("pad"));
+    }
+    SCL_CHECK(Ops<R>::c2r3d(*pb, B, fsb, 6));
+    // smag_dealias.f90:7,44-57: c1 = 0.026 (real(sp)), const = 1/(padded points)
+    const double c1 = (double)(float)0.026f;
+    const double coef = -2.0 * c1 * delta * delta / ((double)pb->g.nx * pb->g.ny * pb->g.nz);
+    hipLaunchKernelGGL((k_smagd_phys<R>), grid1(pb->g.nmain), dim3(256), 0, p->stream, B, fsb, pb->g.nmain, (R)coef);
+    SCL_CHECK(check_launch("smagd_phys"));
+    SCL_CHECK(Ops<R>::r2c3d(*pb, B, fsb, 6));
+    void* out[6] = {t11, t12, t13, t22, t23, t33};
+    for (int q = 0; q < 6; ++q) {
+      hipLaunchKernelGGL((k_unpad<R>), grid1(p->g.ntot), dim3(256), 0, p->stream, B + q * fsb, pb->g, S, p->g);
+      SCL_CHECK(check_launch("unpad"));
+      SCL_CHECK(download<R>(p, S, out[q]));
+    }
+    return SCL_OK;
   });
 }
 

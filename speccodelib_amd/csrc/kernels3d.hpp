@@ -986,6 +986,74 @@ __device__ inline void split_xyz(const Geom& g, int64_t i, int& x, int& y, int& 
   g.yz(row, y, z);
 }
 
+// padd (padd.f90:9-13) on the device, split layouts: every entry of the
+// padded field gb is written, with the entry of g at the same signed
+// wavenumber (kx <= nx/2, -ny/2 <= ky < ny/2, -nz/2 <= kz < nz/2) or 0.
+template <class R>
+__global__ __launch_bounds__(256) void k_pad(const cpx<R>* __restrict__ u, Geom g, cpx<R>* __restrict__ ub, Geom gb) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= gb.ntot) return;
+  int x, y, z;
+  split_xyz(gb, i, x, y, z);
+  const int ky = kwave(y, gb.ny), kz = kwave(z, gb.nz);
+  cpx<R> v = mk<R>(0, 0);
+  if (x <= g.mx && ky >= -g.ny / 2 && ky < g.ny / 2 && kz >= -g.nz / 2 && kz < g.nz / 2) {
+    const int64_t row = g.row((ky + g.ny) % g.ny, (kz + g.nz) % g.nz);
+    v = u[x < g.mx ? row * g.mx + x : g.nmain + row];
+  }
+  ub[i] = v;
+}
+
+// unpadd (unpadd.f90:9-12): every entry of u from the padded entry at its
+// signed wavenumber.
+template <class R>
+__global__ __launch_bounds__(256) void k_unpad(const cpx<R>* __restrict__ ub, Geom gb, cpx<R>* __restrict__ u, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z;
+  split_xyz(g, i, x, y, z);
+  const int64_t row = gb.row((kwave(y, g.ny) + gb.ny) % gb.ny, (kwave(z, g.nz) + gb.nz) % gb.nz);
+  u[i] = x < gb.mx ? ub[row * gb.mx + x] : ub[gb.nmain + row];
+}
+
+// The strain of smag_dealias.f90:21-26 as written there (off-diagonal
+// terms without the 1/2 of the symmetric strain): 11, 12, 13, 22, 23, 33
+// into S[q * fs].
+template <class R>
+__global__ __launch_bounds__(256) void k_strain_smagd(const cpx<R>* __restrict__ v, cpx<R>* __restrict__ S, int64_t fs,
+                                                      Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.ntot) return;
+  int x, y, z;
+  split_xyz(g, i, x, y, z);
+  const R kx = (R)x, ky = (R)kwave(y, g.ny), kz = (R)kwave(z, g.nz);
+  const cpx<R> a = v[i], b = v[i + fs], c = v[i + 2 * fs];
+  S[i] = mul_i(kx * a);
+  S[i + fs] = mul_i(kx * b + ky * a);
+  S[i + 2 * fs] = mul_i(kx * c + kz * a);
+  S[i + 3 * fs] = mul_i(ky * b);
+  S[i + 4 * fs] = mul_i(ky * c + kz * b);
+  S[i + 5 * fs] = mul_i(kz * c);
+}
+
+// smag_dealias.f90:28-50 on the padded physical fields (packed reals of the
+// main rows): |S| = sqrt(2 (S11^2 + S22^2 + S33^2 + 2 (S12^2 + S13^2 + S23^2)))
+// and S_q <- -2 c1 delta^2 |S| S_q * scale, in place.
+template <class R>
+__global__ __launch_bounds__(256) void k_smagd_phys(cpx<R>* __restrict__ S, int64_t fs, int64_t n, R coef) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  cpx<R> s[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) s[q] = S[i + q * fs];
+  const R ax = sqrt((R)2 * (s[0].x * s[0].x + s[3].x * s[3].x + s[5].x * s[5].x +
+                            (R)2 * (s[1].x * s[1].x + s[2].x * s[2].x + s[4].x * s[4].x)));
+  const R ay = sqrt((R)2 * (s[0].y * s[0].y + s[3].y * s[3].y + s[5].y * s[5].y +
+                            (R)2 * (s[1].y * s[1].y + s[2].y * s[2].y + s[4].y * s[4].y)));
+#pragma unroll
+  for (int q = 0; q < 6; ++q) S[i + q * fs] = mk<R>(coef * ax * s[q].x, coef * ay * s[q].y);
+}
+
 // symmetrize.f90:41-48 zeroing (the Hermitian copy is k_herm_fix), and,
 // when thr > 0, dealiasing.f90:35's k2 >= thr truncation.
 template <class R>

@@ -28,6 +28,7 @@ struct scl_plan3d {
   void* tw = nullptr;  // twiddle table exp(-2 pi i j/L), j < L, in the plan's precision
   int L = 0;
   void* scratch = nullptr;  // host-array entry points: 7 fields (6 work + 1 staging)
+  scl_plan3d* padded = nullptr;  // the padded grid of the 3/2-rule products (made on first use)
   size_t scratch_fields = 0;
   double* coef = nullptr;     // forcing coefficients (8 doubles)
   void* diag = nullptr;       // diagnostics partial sums
