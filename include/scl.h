@@ -357,13 +357,27 @@ typedef struct scl_ens2d_params {
   int32_t forced;
   int32_t per_member;
   int32_t use_graph;
+  /* scheme 0: spectral_dns_2d.f90's RHS (2/3 rule) stepped with IF-RK4,
+   * the fields above.  scheme 1: turb_2d.f90 at one rank (turb_2d.f90:
+   * 287-483): phase-shift dealiased nonlinear term with the 8/9 cut, rate
+   * (k sigma)^16 + (k/nu)^-4, forcing etak/conj(w) on kf <= |k| <= kf + 3
+   * (etak = eta/104), Euler and midpoint half steps then IF-AB2; members
+   * are that file's wz(my, mx2) arrays.  Scheme 1 uses only dt, forced,
+   * per_member, use_graph and the four fields below. */
+  int32_t scheme;
+  double sigma;
+  double nu;
+  double eta;
+  double kf;
 } scl_ens2d_params;
 
 /* ngroups groups: sizes[g] x sizes[g] grids, members[g] members each. */
 int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t *sizes, const int32_t *members,
                          const scl_ens2d_params *params, scl_ens2d **ens);
 int32_t scl_ens2d_destroy(scl_ens2d *ens);
-/* Host copy in/out of member j of group g (reference layout). */
+/* Host copy in/out of member j of group g: scheme 0 the reference's
+ * w(lx1, ly); scheme 1 turb_2d.f90's wz(my, mx2) (column kx = 0 .. n/2-1
+ * of my entries each), symmetrized as turb_2d.f90:220 on the way in. */
 int32_t scl_ens2d_set_member(scl_ens2d *ens, int32_t group, int32_t member, const void *w);
 int32_t scl_ens2d_get_member(scl_ens2d *ens, int32_t group, int32_t member, void *w);
 /* Seeded random-phase vorticity (initialize.f90:465-500 draw order: one
@@ -371,7 +385,8 @@ int32_t scl_ens2d_get_member(scl_ens2d *ens, int32_t group, int32_t member, void
  * E(k) ~ k exp(-(k/k0)^2) and scaled to sum' |w|^2/k^2 = energy.  Member q
  * in group-major order draws from seed0 - q (seed0 < 0). */
 int32_t scl_ens2d_initialize(scl_ens2d *ens, int32_t seed0, double k0, double energy);
-/* Sets member j of group g to w, evaluates N for it, copies N to rhs. */
+/* Sets member j of group g to w, evaluates N for it, copies N to rhs
+ * (scheme 1: vy of turb_2d.f90:428-429, in its layout and convention). */
 int32_t scl_ens2d_rhs(scl_ens2d *ens, int32_t group, int32_t member, const void *w, void *rhs);
 int32_t scl_ens2d_step(scl_ens2d *ens, int32_t nsteps);
 int32_t scl_ens2d_sync(scl_ens2d *ens);

@@ -100,6 +100,11 @@ class ScEns2DParams(ctypes.Structure):
         ("forced", ctypes.c_int32),
         ("per_member", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("scheme", ctypes.c_int32),
+        ("sigma", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("eta", ctypes.c_double),
+        ("kf", ctypes.c_double),
     ]
 
 
@@ -516,19 +521,26 @@ class DNS:
 
 
 class Ensemble2D:
-    """Batched 2D vorticity ensembles (spectral_dns_2d.f90 time loop, IF-RK4).
+    """Batched 2D vorticity ensembles.
 
-    ``groups`` is a list of ``(n, members)``; a member's field is the
-    reference's ``complex(sp), dimension(lx1, ly)``, i.e. a numpy array of
-    shape ``(n, n//2 + 1)``.  ``per_member=True`` gives every member its own
-    launches (the comparison case)."""
+    scheme 0 (default): spectral_dns_2d.f90's RHS stepped with IF-RK4; a
+    member's field is the reference's ``complex(sp), dimension(lx1, ly)``,
+    i.e. a numpy array of shape ``(n, n//2 + 1)``.
+    scheme 1: turb_2d.f90 at one rank (phase-shift dealiasing, (k sigma)^16
+    + (k/nu)^-4, forcing eta/104/conj(w) on kf <= |k| <= kf + 3, its
+    half-step bootstrap then IF-AB2); a member is that file's
+    ``wz(my, mx2)`` as an array of shape ``(n, n//2)`` indexed [iy, ix].
+    ``groups`` is a list of ``(n, members)``.  ``per_member=True`` gives
+    every member its own launches (the comparison case)."""
 
     def __init__(self, groups, prec=SP, dt=5e-4, c_hyper=1e-5, n_hyper=8, c_hypo=0.0, n_hypo=0, damp=0.1,
-                 forced=True, fkmin=4.0, fkmax=5.0, eps=0.01, per_member=False, use_graph=True):
+                 forced=True, fkmin=4.0, fkmax=5.0, eps=0.01, per_member=False, use_graph=True, scheme=0,
+                 sigma=0.0, nu=1.0, eta=0.0, kf=4.0):
         self.groups = [(int(n), int(m)) for n, m in groups]
         self.prec = prec
+        self.scheme = int(scheme)
         self.params = ScEns2DParams(dt, c_hyper, c_hypo, damp, fkmin, fkmax, eps, int(n_hyper), int(n_hypo),
-                                    int(forced), int(per_member), int(use_graph))
+                                    int(forced), int(per_member), int(use_graph), self.scheme, sigma, nu, eta, kf)
         sizes = np.array([n for n, _ in self.groups], np.int32)
         mem = np.array([m for _, m in self.groups], np.int32)
         h = ctypes.c_void_p()
@@ -545,26 +557,40 @@ class Ensemble2D:
 
     def shape(self, g):
         n = self.groups[g][0]
-        return (n, n // 2 + 1)
+        return (n, n // 2) if self.scheme == 1 else (n, n // 2 + 1)
+
+    def _in(self, g, w):
+        if self.scheme == 1:  # Fortran order of wz(my, mx2): iy fastest
+            if w.dtype != _cdtype(self.prec) or w.shape != self.shape(g):
+                raise ValueError(f"expected {np.dtype(_cdtype(self.prec))} array of shape {self.shape(g)}")
+            return np.ascontiguousarray(w.T)
+        return _field(w, self.prec, self.shape(g))
+
+    def _out(self, g):
+        s = self.shape(g)
+        return np.zeros(s[::-1] if self.scheme == 1 else s, _cdtype(self.prec))
+
+    def _view(self, out):
+        return out.T if self.scheme == 1 else out
 
     def set_member(self, g, j, w):
-        _field(w, self.prec, self.shape(g))
-        _check(lib().scl_ens2d_set_member(self.h, g, j, _ptr(w)))
+        a = self._in(g, w)  # held until the call returns
+        _check(lib().scl_ens2d_set_member(self.h, g, j, _ptr(a)))
 
     def get_member(self, g, j):
-        out = np.zeros(self.shape(g), _cdtype(self.prec))
+        out = self._out(g)
         _check(lib().scl_ens2d_get_member(self.h, g, j, _ptr(out)))
-        return out
+        return self._view(out)
 
     def initialize(self, seed0=-1, k0=8.0, energy=0.5):
         _check(lib().scl_ens2d_initialize(self.h, seed0, k0, energy))
 
     def rhs(self, g, j, w):
         """Sets member j of group g to w and returns N(w)."""
-        _field(w, self.prec, self.shape(g))
-        out = np.zeros(self.shape(g), _cdtype(self.prec))
-        _check(lib().scl_ens2d_rhs(self.h, g, j, _ptr(w), _ptr(out)))
-        return out
+        out = self._out(g)
+        a = self._in(g, w)
+        _check(lib().scl_ens2d_rhs(self.h, g, j, _ptr(a), _ptr(out)))
+        return self._view(out)
 
     def step(self, n=1):
         _check(lib().scl_ens2d_step(self.h, n))
@@ -674,7 +700,8 @@ def padd(u, shape_b):
     prec = SP if u.dtype == np.complex64 else DP
     ub = np.zeros(shape_b, u.dtype)
     lz, ly, lx1 = u.shape
-    _check(lib().scl_padd(prec, _ptr(np.ascontiguousarray(u)), _ptr(ub), lx1, ly, lz, shape_b[2], shape_b[1],
+    u = np.ascontiguousarray(u)  # held until the call returns
+    _check(lib().scl_padd(prec, _ptr(u), _ptr(ub), lx1, ly, lz, shape_b[2], shape_b[1],
                           shape_b[0]))
     return ub
 
@@ -684,7 +711,8 @@ def unpadd(ub, shape):
     prec = SP if ub.dtype == np.complex64 else DP
     u = np.zeros(shape, ub.dtype)
     lzb, lyb, lxb1 = ub.shape
-    _check(lib().scl_unpadd(prec, _ptr(np.ascontiguousarray(ub)), _ptr(u), shape[2], shape[1], shape[0], lxb1, lyb,
+    ub = np.ascontiguousarray(ub)
+    _check(lib().scl_unpadd(prec, _ptr(ub), _ptr(u), shape[2], shape[1], shape[0], lxb1, lyb,
                             lzb))
     return u
 

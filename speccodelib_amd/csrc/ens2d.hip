@@ -40,7 +40,10 @@ struct Group {
   }
 };
 
-enum { F_W = 0, F_WS = 3, F_ACC = 4, F_S = 5, NFIELDS = 6 };
+// Scheme 1 (turb_2d.f90) keeps the previous nonlinear term in F_ACC, the
+// state before the first step in F_S and the unshifted half of the
+// nonlinear term in a seventh field, F_NA.
+enum { F_W = 0, F_WS = 3, F_ACC = 4, F_S = 5, F_NA = 6, NFIELDS = 6, NFIELDS_TURB = 7 };
 
 // FP64 plans keep one schedule (E = 8, 8 columns) and lines up to 256.
 template <class R, int N, int E, int CW>
@@ -170,6 +173,29 @@ int64_t row_grid_size(const scl_ens2d& e, const Group& gr) {
   return gr.row_grid ? ngroups : std::min<int64_t>(ngroups, (int64_t)resident * e.ncu);
 }
 
+template <class R>
+int32_t yinv_launch(const Group& gr, const cpx<R>* src, int shift, hipStream_t st) {
+  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(gr.tw);
+  return with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
+    constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
+    hipLaunchKernelGGL((k_yinv2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
+                       src, gr.field<R>(F_W), gr.fs(), gr.g, tw, 1, shift);
+    return check_launch("yinv2d");
+  });
+}
+
+template <class R>
+int32_t row_launch(const scl_ens2d& e, const Group& gr, hipStream_t st) {
+  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(gr.tw);
+  return with_row2d<R>(gr.n / 2, [&](auto MC) -> int32_t {
+    constexpr int M = decltype(MC)::value;
+    const R scale = (R)1 / (R)((double)gr.n * gr.n);
+    hipLaunchKernelGGL((k_row_adv2d<R, M>), dim3((unsigned)row_grid_size<R, M>(e, gr)), dim3(RowCfg<R, M>::BS), 0,
+                       st, gr.field<R>(F_W), gr.fs(), gr.g, scale, tw, 2, 1);
+    return check_launch("row_adv2d");
+  });
+}
+
 // One RHS evaluation (mode 0) or IF-RK4 stage (modes 1..4) of a group.
 // primed: W[0..2] already hold the y-inverse of the stage vorticity (the
 // previous stage was fused); fuse: the y-forward also y-inverses the new
@@ -186,21 +212,8 @@ int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st, boo
   const int64_t fs = gr.fs();
   const Rhs2D<R> rp = rhs_params<R>(e, gr.n);
   if (fuse && mode < 1) return set_error("fused 2D stage: mode %d is not a stage 1..4", mode), SCL_EINVAL;
-  if (!primed) {
-    SCL_CHECK(with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
-      constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
-      hipLaunchKernelGGL((k_yinv2d<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0, st,
-                         sin, W, fs, gr.g, tw, 1);
-      return check_launch("yinv2d");
-    }));
-  }
-  SCL_CHECK(with_row2d<R>(gr.n / 2, [&](auto MC) -> int32_t {
-    constexpr int M = decltype(MC)::value;
-    const R scale = (R)1 / (R)((double)gr.n * gr.n);
-    hipLaunchKernelGGL((k_row_adv2d<R, M>), dim3((unsigned)row_grid_size<R, M>(e, gr)), dim3(RowCfg<R, M>::BS), 0,
-                       st, W, fs, gr.g, scale, tw, 2, 1);
-    return check_launch("row_adv2d");
-  }));
+  if (!primed) SCL_CHECK(yinv_launch<R>(gr, sin, 0, st));
+  SCL_CHECK(row_launch<R>(e, gr, st));
   return with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
     constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
     const dim3 grd(col_grid<CW, 1>(gr.g)), blk(Col2D<R, N, E, CW>::BS);
@@ -213,13 +226,56 @@ int32_t stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st, boo
   });
 }
 
+template <class R>
+Turb2DP<R> turb_params(const scl_ens2d& e, int n) {
+  Turb2DP<R> t;
+  t.dt = (R)e.p.dt;
+  t.sigma = (R)e.p.sigma;
+  t.nu = (R)e.p.nu;
+  const R etak = (R)e.p.eta / (R)104;  // turb_2d.f90:111
+  t.etak4 = etak / (R)4;
+  t.kf = (R)e.p.kf;
+  // dealiasing(): ass = 2.0/9.0*(my*my) in single precision (turb_2d.f90:519)
+  t.kc2 = (R)((2.0f / 9.0f) * (float)((double)n * n));
+  t.forced = e.p.forced;
+  return t;
+}
+
+// One turb_2d.f90 step (mode TURB_STEP0, TURB_STEP1, TURB_AB2) or nonlinear
+// term (TURB_RHS) of a group: the unshifted half, then the shifted half and
+// the update.
+template <class R>
+int32_t turb_stage(const scl_ens2d& e, const Group& gr, int mode, hipStream_t st) {
+  using C = cpx<R>;
+  const C* tw = reinterpret_cast<const C*>(gr.tw);
+  const Turb2DP<R> tp = turb_params<R>(e, gr.n);
+  for (int half = 0; half < 2; ++half) {
+    SCL_CHECK(yinv_launch<R>(gr, gr.field<R>(F_WS), half, st));
+    SCL_CHECK(row_launch<R>(e, gr, st));
+    SCL_CHECK(with_col2d<R>(gr.n, gr.e, gr.cw, [&](auto NC, auto EC, auto CC) -> int32_t {
+      constexpr int N = decltype(NC)::value, E = decltype(EC)::value, CW = decltype(CC)::value;
+      hipLaunchKernelGGL((k_yfwd2d_turb<R, N, E, CW>), dim3(col_grid<CW, 1>(gr.g)), dim3(Col2D<R, N, E, CW>::BS), 0,
+                         st, gr.field<R>(F_W), gr.fs(), gr.field<R>(F_NA), gr.field<R>(F_WS), gr.field<R>(F_ACC),
+                         gr.field<R>(F_S), half ? mode : (int)TURB_HALF, tp, gr.g, tw, 1);
+      return check_launch("yfwd2d_turb");
+    }));
+  }
+  return SCL_OK;
+}
+
+int turb_mode(int64_t step) { return step == 0 ? TURB_STEP0 : step == 1 ? TURB_STEP1 : TURB_AB2; }
+
 // nsteps steps of a group as one stage chain: the first stage runs its own
 // y-inverse, every later one gets it from the previous stage's fused
 // y-forward, and the last stage is not fused (the chain leaves no state in
 // W).  The kx = 0 column is made Hermitian in the same workgroup, so the
 // chain gives the same results as separate stages.
 template <class R>
-int32_t group_steps(const scl_ens2d& e, const Group& gr, hipStream_t st, int nsteps) {
+int32_t group_steps(const scl_ens2d& e, const Group& gr, hipStream_t st, int nsteps, int64_t first) {
+  if (e.p.scheme == 1) {
+    for (int s = 0; s < nsteps; ++s) SCL_CHECK(turb_stage<R>(e, gr, turb_mode(first + s), st));
+    return SCL_OK;
+  }
   for (int s = 0; s < nsteps; ++s)
     for (int mode = 1; mode <= 4; ++mode)
       SCL_CHECK(stage<R>(e, gr, mode, st, s > 0 || mode > 1, s < nsteps - 1 || mode < 4));
@@ -230,13 +286,15 @@ int32_t group_steps(const scl_ens2d& e, const Group& gr, hipStream_t st, int nst
 // runs its nsteps steps as one chain on its own branch (fork/join through
 // events, which graph capture turns into graph edges) and the branches
 // join once, at the end.
-int32_t launch_steps(scl_ens2d& e, int nsteps) {
+// first: the index of the first step (scheme 1 bootstraps on steps 0, 1).
+int32_t launch_steps(scl_ens2d& e, int nsteps, int64_t first) {
   const bool fork = !e.p.per_member && e.groups.size() > 1;
   if (fork) SCL_HIP(hipEventRecord(e.fork, e.stream));
   for (auto& gr : e.groups) {
     hipStream_t st = fork ? gr.stream : e.stream;
     if (fork) SCL_HIP(hipStreamWaitEvent(st, e.fork, 0));
-    SCL_CHECK(e.prec == SCL_SP ? group_steps<float>(e, gr, st, nsteps) : group_steps<double>(e, gr, st, nsteps));
+    SCL_CHECK(e.prec == SCL_SP ? group_steps<float>(e, gr, st, nsteps, first)
+                               : group_steps<double>(e, gr, st, nsteps, first));
     if (fork) {
       SCL_HIP(hipEventRecord(gr.done, st));
       SCL_HIP(hipStreamWaitEvent(e.stream, gr.done, 0));
@@ -254,11 +312,12 @@ void drop_graph(scl_ens2d& e) {
   }
 }
 
-// Graph i of e (0: one step, 1: a chunk of e.chunk steps), built on first use.
+// Graph i of e (0: one step, 1: a chunk of e.chunk steps), built on first
+// use; scheme 1 graphs hold IF-AB2 steps only.
 int32_t graph_exec(scl_ens2d& e, int i, hipGraphExec_t* out) {
   if (!e.exec[i]) {
     SCL_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    const int32_t st = launch_steps(e, i ? e.chunk : 1);
+    const int32_t st = launch_steps(e, i ? e.chunk : 1, 2);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e.stream, &g);
     if (st != SCL_OK) {
@@ -273,8 +332,18 @@ int32_t graph_exec(scl_ens2d& e, int i, hipGraphExec_t* out) {
   return SCL_OK;
 }
 
+void advance_clock(scl_ens2d& e, int nsteps) {
+  for (int s = 0; s < nsteps; ++s, ++e.steps)
+    e.time += (e.p.scheme == 1 && e.steps < 2) ? 0.5 * e.p.dt : e.p.dt;  // turb_2d.f90:457,467,481
+}
+
 int32_t run_steps(scl_ens2d& e, int nsteps) {
   if (nsteps < 0) return set_error("nsteps < 0"), SCL_EINVAL;
+  // scheme 1: the two bootstrap steps run eagerly
+  for (; nsteps > 0 && e.p.scheme == 1 && e.steps < 2; --nsteps) {
+    SCL_CHECK(launch_steps(e, 1, e.steps));
+    advance_clock(e, 1);
+  }
   int done = 0;
   if (e.p.use_graph) {
     hipGraphExec_t x;
@@ -287,11 +356,10 @@ int32_t run_steps(scl_ens2d& e, int nsteps) {
       for (; done < nsteps; ++done) SCL_HIP(hipGraphLaunch(x, e.stream));
     }
   } else {
-    SCL_CHECK(launch_steps(e, nsteps));
+    SCL_CHECK(launch_steps(e, nsteps, e.steps));
     done = nsteps;
   }
-  e.steps += done;
-  e.time += done * e.p.dt;
+  advance_clock(e, done);
   return SCL_OK;
 }
 
@@ -326,6 +394,57 @@ int32_t copy_member(const scl_ens2d& e, const Group& gr, int b, int f, void* hos
   }
   SCL_HIP(hipStreamSynchronize(e.stream));
   return SCL_OK;
+}
+
+// turb_2d.f90's wz(my, mx2) (C = 2 conj(D), see Turb2DP) <-> the host
+// reference layout (lx1, ly) of the normalised coefficients D.  On the way
+// in, symmetrize (turb_2d.f90:569-590): row ky = -n/2 and k = 0 zeroed, the
+// kx = 0 column averaged with its mirror for 1 <= y <= n/2 - 2.
+template <class R>
+void turb_in(const cpx<R>* wz, int n, std::vector<cpx<R>>& h) {
+  const int mx2 = n / 2, lx1 = mx2 + 1;
+  h.assign((size_t)lx1 * n, mk<R>(0, 0));
+  for (int x = 0; x < mx2; ++x)
+    for (int y = 0; y < n; ++y) h[(size_t)y * lx1 + x] = (R)0.5 * conj(wz[(size_t)x * n + y]);
+  for (int x = 0; x < lx1; ++x) h[(size_t)(n / 2) * lx1 + x] = mk<R>(0, 0);
+  h[0] = mk<R>(0, 0);
+  for (int y = 1; y <= n / 2 - 2; ++y)
+    h[(size_t)y * lx1] = (R)0.5 * (h[(size_t)y * lx1] + conj(h[(size_t)(n - y) * lx1]));
+  for (int y = 1; y <= n / 2 - 2; ++y) h[(size_t)(n - y) * lx1] = conj(h[(size_t)y * lx1]);
+}
+
+template <class R>
+void turb_out(const std::vector<cpx<R>>& h, int n, cpx<R>* wz) {
+  const int mx2 = n / 2, lx1 = mx2 + 1;
+  for (int x = 0; x < mx2; ++x)
+    for (int y = 0; y < n; ++y) wz[(size_t)x * n + y] = (R)2 * conj(h[(size_t)y * lx1 + x]);
+}
+
+template <class F>
+int32_t with_prec(int prec, F&& f) {
+  return prec == SCL_SP ? f(float{}) : f(double{});
+}
+
+// Member copies in the layout of the ensemble's scheme.
+int32_t put_member(const scl_ens2d& e, const Group& gr, int b, int f, const void* w) {
+  if (e.p.scheme != 1) return copy_member(e, gr, b, f, const_cast<void*>(w), true);
+  return with_prec(e.prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    std::vector<cpx<R>> h;
+    turb_in<R>(static_cast<const cpx<R>*>(w), gr.n, h);
+    return copy_member(e, gr, b, f, h.data(), true);
+  });
+}
+
+int32_t take_member(const scl_ens2d& e, const Group& gr, int b, int f, void* w) {
+  if (e.p.scheme != 1) return copy_member(e, gr, b, f, w, false);
+  return with_prec(e.prec, [&](auto zero) -> int32_t {
+    using R = decltype(zero);
+    std::vector<cpx<R>> h((size_t)(gr.n / 2 + 1) * gr.n);
+    SCL_CHECK(copy_member(e, gr, b, f, h.data(), false));
+    turb_out<R>(h, gr.n, static_cast<cpx<R>*>(w));
+    return SCL_OK;
+  });
 }
 
 // Seeded vorticity of one member, computed in FP64 (reference layout).
@@ -420,6 +539,10 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
   }
   if (!(params->dt > 0)) return set_error("dt must be > 0"), SCL_EINVAL;
   if (params->n_hyper < 0 || params->n_hypo < 0) return set_error("negative viscosity order"), SCL_EINVAL;
+  if (params->scheme != 0 && params->scheme != 1) return set_error("scheme %d: 0 or 1", params->scheme), SCL_EINVAL;
+  if (params->scheme == 1 && !(std::isfinite(params->sigma) && std::isfinite(params->nu) && params->nu != 0 &&
+                               std::isfinite(params->eta) && std::isfinite(params->kf)))
+    return set_error("scheme 1: sigma, nu (nonzero), eta and kf must be finite"), SCL_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     (void)hipGetLastError();
@@ -471,7 +594,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
       gr.tw = tw;
       gr.e = (prec == SCL_SP && n > 512) ? 16 : 8;
       gr.cw = 8;
-      const size_t bytes = (size_t)NFIELDS * gr.fs() * es;
+      const size_t bytes = (size_t)(params->scheme == 1 ? NFIELDS_TURB : NFIELDS) * gr.fs() * es;
       if (hipMalloc(&gr.base, bytes) != hipSuccess) {
         (void)hipGetLastError();
         e->groups.push_back(gr);
@@ -517,7 +640,11 @@ extern "C" int32_t scl_ens2d_set_member(scl_ens2d* e, int32_t group, int32_t mem
   const Group* gr;
   int b;
   SCL_CHECK(locate(*e, group, member, &gr, &b));
-  return copy_member(*e, *gr, b, F_WS, const_cast<void*>(w), true);
+  SCL_CHECK(put_member(*e, *gr, b, F_WS, w));
+  // scheme 1: a new state starts turb_2d.f90's bootstrap again (its restart
+  // reads wz and runs istep = 0, 1 first)
+  if (e->p.scheme == 1) e->steps = 0;
+  return SCL_OK;
 }
 
 extern "C" int32_t scl_ens2d_get_member(scl_ens2d* e, int32_t group, int32_t member, void* w) {
@@ -525,7 +652,7 @@ extern "C" int32_t scl_ens2d_get_member(scl_ens2d* e, int32_t group, int32_t mem
   const Group* gr;
   int b;
   SCL_CHECK(locate(*e, group, member, &gr, &b));
-  return copy_member(*e, *gr, b, F_WS, w, false);
+  return take_member(*e, *gr, b, F_WS, w);
 }
 
 extern "C" int32_t scl_ens2d_initialize(scl_ens2d* e, int32_t seed0, double k0, double energy) {
@@ -551,7 +678,13 @@ extern "C" int32_t scl_ens2d_rhs(scl_ens2d* e, int32_t group, int32_t member, co
   const Group* gr;
   int b;
   SCL_CHECK(locate(*e, group, member, &gr, &b));
-  SCL_CHECK(copy_member(*e, *gr, b, F_WS, const_cast<void*>(w), true));
+  SCL_CHECK(put_member(*e, *gr, b, F_WS, w));
+  if (e->p.scheme == 1) {
+    e->steps = 0;
+    SCL_CHECK(e->prec == SCL_SP ? turb_stage<float>(*e, *gr, TURB_RHS, e->stream)
+                                : turb_stage<double>(*e, *gr, TURB_RHS, e->stream));
+    return take_member(*e, *gr, b, F_NA, rhs);
+  }
   SCL_CHECK(e->prec == SCL_SP ? stage<float>(*e, *gr, 0, e->stream) : stage<double>(*e, *gr, 0, e->stream));
   return copy_member(*e, *gr, b, F_W, rhs, false);
 }

@@ -112,11 +112,24 @@ __device__ inline void yinv2d_store(cpx<R> (&o)[E], cpx<R>* lds, const cpx<R>* t
   store(o, 2);
 }
 
-// YINV.  s: the stage vorticity; w: three work fields (u, v, w) at stride fstride.
+// exp(i pi s/n) for s = kx + ky: s/n is a dyadic rational, exact in R.
+template <class R>
+__device__ inline cpx<R> half_cell_phase(int s, int n) {
+  R sn, cs;
+  if constexpr (sizeof(R) == 4) sincospif((float)s / (float)n, &sn, &cs);
+  else sincospi((double)s / (double)n, &sn, &cs);
+  return mk<R>(cs, sn);
+}
+
+// YINV.  s: the stage vorticity; w: three work fields (u, v, w) at stride
+// fstride.  shift: the fields are taken at points moved by half a cell along
+// x and y, the spectrum times exp(-i pi (kx + ky)/N) (the phase shift of
+// turb_2d.f90:369-371, in the normalised coefficients of the real field).
 template <class R, int N, int E, int CW>
 __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yinv2d(const cpx<R>* __restrict__ s,
                                                                    cpx<R>* __restrict__ w, int64_t fstride, Geom g,
-                                                                   const cpx<R>* __restrict__ tw, int tws) {
+                                                                   const cpx<R>* __restrict__ tw, int tws,
+                                                                   int shift) {
   using CC = Col2D<R, N, E, CW>;
   constexpr int T = CC::T;
   __shared__ cpx<R> lds[lds_pad(N) * CW];
@@ -127,6 +140,10 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yinv2d(const cpx<R
   cpx<R> o[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) o[m] = t.ok ? s[t.base + (int64_t)(rt + T * m) * t.rs] : mk<R>(0, 0);
+  if (shift) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) o[m] = o[m] * conj(half_cell_phase<R>(t.kx + kwave(rt + T * m, N), N));
+  }
   yinv2d_store<R, N, E, CW>(o, lds, twl, rt, c, t, w, fstride);
 }
 
@@ -321,6 +338,131 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
   if constexpr (FUSED) {
     if (kx0_tile) __syncthreads();  // the staged kx = 0 column is read
     yinv2d_store<R, N, E, CW>(ns, lds, twl, rt, c, t, W, fstride);
+  }
+}
+
+// turb_2d.f90's step (one rank), in the normalised coefficients D of the
+// real fields; its own coefficients are C = 2 conj(D) (Ooura transforms,
+// exp(+i k.x) forward, kx = n/2 dropped), so i k C <-> -i k D, the phase
+// exp(i phi) C <-> exp(-i phi) D and etak/conj(C) <-> (etak/4)/conj(D).
+template <class R>
+struct Turb2DP {
+  R dt;
+  R sigma, nu;  // L = (k sigma)^16 + (k/nu)^-4
+  R etak4;      // etak/4, etak = eta/104
+  R kf;         // forced band kf <= |k| <= kf + 3
+  R kc2;        // dealiasing: zero where k2 >= (2/9) n^2
+  int forced;
+};
+
+enum { TURB_HALF = 0, TURB_STEP0 = 1, TURB_STEP1 = 2, TURB_AB2 = 3, TURB_RHS = 4 };
+
+// YFWD of one half of the phase-shift dealiased nonlinear term
+// (turb_2d.f90:287-421) and the stepper (:430-483).  W: the two product
+// fields; NA: the unshifted half; WS: state w; OWZ: the previous nonlinear
+// term owz; W0: the state before the first step.
+//   TURB_HALF: NA = -(i/2)(kx F + ky G) of the unshifted products;
+//   otherwise N = NA - (i/2) exp(i phi)(kx F + ky G) of the shifted ones,
+//   zeroed on k2 >= kc2, ky = -n/2, kx = n/2 and k = 0 (dealiasing and
+//   symmetrize, :428-429), then with f(a) = etak4/conj(a) in the band:
+//   TURB_STEP0: W0 = w; OWZ = N; w = w (1 - L dt/2) + dt/2 (N + f(w))
+//   TURB_STEP1: w = W0 + dt (N + f(w)) - L dt w
+//   TURB_AB2:   w = [w + dt/2 (3 (N + f(w)) - E (owz + f(owz)))] E,
+//               E = exp(-L dt) (1 at k = 0); OWZ = N
+//   TURB_RHS:   NA = N.
+// The kx = 0 column of N and of the new w is made Hermitian as in k_yfwd2d.
+template <class R, int N, int E, int CW>
+__global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d_turb(
+    const cpx<R>* W, int64_t fstride, cpx<R>* NA, cpx<R>* WS, cpx<R>* OWZ, cpx<R>* W0, int mode, Turb2DP<R> p, Geom g,
+    const cpx<R>* __restrict__ tw, int tws) {
+  using CC = Col2D<R, N, E, CW>;
+  constexpr int T = CC::T;
+  __shared__ cpx<R> lds[lds_pad(N) * CW];
+  __shared__ cpx<R> twl[N];
+  for (int i = threadIdx.x; i < N; i += CC::BS) twl[i] = tw[i * tws];
+  const int c = threadIdx.x % CW, rt = threadIdx.x / CW;
+  const int wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int xtn = g.mx / CW;
+  const bool kx0_tile = wg < xtn * g.nz && wg % xtn == 0;
+  const Tile t = col_tile<CW, 1>(g, wg, c);
+  using L = Line<R, N, E, -1>;
+  cpx<R> f0[E], f1[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int64_t o = t.base + (int64_t)(rt + T * m) * t.rs;
+    f0[m] = t.ok ? W[o] : mk<R>(0, 0);
+    f1[m] = t.ok ? W[o + fstride] : mk<R>(0, 0);
+  }
+  L::run(f0, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(f1, lds, rt, twl, 1, ColIdx<CW>{c});
+  if (kx0_tile) __syncthreads();  // the transforms' LDS reads are done
+  const int ikx = t.kx;
+  const R kx = (R)ikx;
+  auto force = [&](cpx<R> a) {
+    const R den = a.x * a.x + a.y * a.y;
+    return mk<R>(p.etak4 * a.x / den, p.etak4 * a.y / den);
+  };
+  auto put = [&](int y, cpx<R> v1, cpx<R> v2) {
+    const int64_t o = t.base + (int64_t)y * t.rs;
+    if (mode == TURB_HALF || mode == TURB_RHS) {
+      NA[o] = v1;
+    } else {
+      WS[o] = v1;
+      if (mode != TURB_STEP1) OWZ[o] = v2;
+    }
+  };
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int y = L::out_pos(m, rt);
+    const int iky = kwave(y, N);
+    const int ik2 = ikx * ikx + iky * iky;
+    const R k2 = ik2 == 0 ? (R)0.5 : (R)ik2;  // wavenumber(): k2(0, 0) = 0.5
+    const int64_t o = t.base + (int64_t)y * t.rs;
+    const cpx<R> d = kx * f0[m] + (R)iky * f1[m];
+    cpx<R> r = mk<R>((R)0.5 * d.y, (R)-0.5 * d.x);  // -(i/2) d
+    cpx<R> o1 = r, o2 = mk<R>(0, 0);
+    if (mode != TURB_HALF) {
+      r = (t.ok ? NA[o] : mk<R>(0, 0)) + r * half_cell_phase<R>(ikx + iky, N);
+      if (y == N / 2 || ikx == g.mx || ik2 == 0 || k2 >= p.kc2) r = mk<R>(0, 0);
+      o1 = r;
+      if (mode != TURB_RHS && t.ok) {
+        const cpx<R> w = WS[o];
+        const R k = sqrt(k2);
+        const R ks = k * p.sigma, kn = k / p.nu;
+        const R lam = ipow(ks, 16) + (R)1 / ipow(kn, 4);
+        const bool band = p.forced && k >= p.kf && k <= p.kf + (R)3;
+        const R dth = (R)0.5 * p.dt;
+        const cpx<R> fw = band ? force(w) : mk<R>(0, 0);
+        if (mode == TURB_STEP0) {
+          o1 = ((R)1 - lam * dth) * w + dth * (r + fw);
+          o2 = r;
+          W0[o] = w;
+        } else if (mode == TURB_STEP1) {
+          o1 = W0[o] + p.dt * (r + fw) - (lam * p.dt) * w;
+        } else {
+          const R ef = ik2 == 0 ? (R)1 : exp(-lam * p.dt);
+          const cpx<R> ow = OWZ[o];
+          const cpx<R> fo = band ? force(ow) : mk<R>(0, 0);
+          o1 = ef * (w + dth * ((R)3 * (r + fw) - ef * (ow + fo)));
+          o2 = r;
+        }
+      }
+    }
+    if (t.ok) put(y, o1, o2);
+    if (kx0_tile && c == 0) {
+      lds[y] = o1;
+      lds[N + y] = o2;
+    }
+  }
+  if (kx0_tile) {
+    __syncthreads();
+    if (c == 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int y = L::out_pos(m, rt);
+        if (y > N / 2) put(y, conj(lds[N - y]), conj(lds[2 * N - y]));
+      }
+    }
   }
 }
 
