@@ -73,6 +73,8 @@ def kind_bytes_f(kinds):
 
 
 def cpu_cores():
+    """The threads the baseline may use: the host's cores available to this
+    process, at most 16 (one GPU's share of the host)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -95,6 +97,7 @@ def cpu_baseline(n, steps, scl):
         "value": steps / dt,
         "unit": "steps/s",
         "cores": cores,
+        "host_cores": os.cpu_count(),
         "kind": "port",
         "sample": f"{steps} IF-RK4 step(s) of the same {n}^3 FP32 forced DNS, oracle (numpy + scipy.fft "
         f"workers={cores}), {dt:.1f} s",
@@ -389,7 +392,8 @@ def main():
     ap.add_argument("--ens2d-member-steps", type=int, default=20)
     ap.add_argument("--no-ens2d", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--heli1024", type=int, default=0, help="also time this many steps of the 1024^3 helical DNS")
+    ap.add_argument("--heli1024", type=int, default=5,
+                    help="also time this many steps of the 1024^3 helical DNS (0: skip)")
     ap.add_argument("--les256", type=int, default=50,
                     help="also time this many steps of the 256^3 LES + scalar (0: skip)")
     ap.add_argument("--no-main", action="store_true", help="skip the 512^3 DNS (profiling runs of the others)")
