@@ -149,7 +149,7 @@ def bench_ens2d(scl, steps, member_steps):
         "hbm_roofline_frac_step": step_bytes / (bms * 1e-3) / HBM_PEAK,
         "ns_per_point_step": bms * 1e6 / sum(m * n * n for n, m in ENS2D_GROUPS),
         "schedules_e_cw_rowgrid": sched,
-        "tuned_ms_per_rhs": [float(x) for x in best],
+        "tuned_ms_per_step": [float(x) for x in best],
         "finite": finite,
         "energy_mean": float(d[:, 0].mean()),
     }

@@ -401,9 +401,10 @@ int32_t scl_ens2d_diagnostics(scl_ens2d *ens, double *out);
  * workgroup per row group). */
 int32_t scl_ens2d_set_schedule(scl_ens2d *ens, int32_t group, int32_t e, int32_t cw, int32_t row_grid);
 int32_t scl_ens2d_get_schedule(scl_ens2d *ens, int32_t group, int32_t *e, int32_t *cw, int32_t *row_grid);
-/* Autotuner: times reps RHS evaluations of every valid schedule of every
- * group on the device, keeps the fastest; best_ms[g] gets its time per RHS
- * (may be NULL).  The state is not changed. */
+/* Autotuner: times reps steps of every valid schedule of every group on the
+ * device, keeps the fastest; best_ms[g] gets its time per step (may be
+ * NULL).  The state is saved before and restored after each candidate, so
+ * it is not changed. */
 int32_t scl_ens2d_tune(scl_ens2d *ens, int32_t reps, double *best_ms);
 
 /* ---------------------------------------------------------------------

@@ -625,7 +625,8 @@ class Ensemble2D:
         return tuple(x.value for x in v)
 
     def tune(self, reps=5):
-        """Autotunes every group's schedule; returns the best ms per RHS per group."""
+        """Autotunes every group's schedule on whole steps (state restored);
+        returns the best ms per step per group."""
         best = np.zeros(len(self.groups))
         _check(lib().scl_ens2d_tune(self.h, reps, _ptr(best)))
         return best

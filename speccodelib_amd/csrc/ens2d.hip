@@ -65,11 +65,12 @@ int32_t with_col2d(int n, int e, int cw, F&& f) {
         return SCL_EINVAL;
       }
     };
+    if (e == 8 && cw == 4) return pick(Int<8>{}, Int<4>{});
     if (e == 8 && cw == 8) return pick(Int<8>{}, Int<8>{});
     if (e == 8 && cw == 16) return pick(Int<8>{}, Int<16>{});
     if (e == 16 && cw == 8) return pick(Int<16>{}, Int<8>{});
     if (e == 16 && cw == 16) return pick(Int<16>{}, Int<16>{});
-    set_error("schedule e=%d cw=%d: e and cw must be 8 or 16", e, cw);
+    set_error("schedule e=%d cw=%d: e must be 8 or 16, cw 8 or 16 (or 4 with e = 8)", e, cw);
     return SCL_EINVAL;
   };
   switch (n) {
@@ -592,7 +593,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
       gr.nb = nb;
       gr.g = make_geom(n, n, nb);
       gr.tw = tw;
-      gr.e = (prec == SCL_SP && n > 512) ? 16 : 8;
+      gr.e = 8;  // at 1024 points: one 1024-thread workgroup per CU, 4 waves per SIMD
       gr.cw = 8;
       const size_t bytes = (size_t)(params->scheme == 1 ? NFIELDS_TURB : NFIELDS) * gr.fs() * es;
       if (hipMalloc(&gr.base, bytes) != hipSuccess) {
@@ -773,45 +774,84 @@ extern "C" int32_t scl_ens2d_get_schedule(scl_ens2d* e, int32_t group, int32_t* 
   return SCL_OK;
 }
 
+// The tuner times whole steps (the chain of fused stages the runs use; an
+// RHS alone ranks the schedules differently at 1024 points), so it saves
+// the fields a step changes for the next one (the state, and for scheme 1
+// the previous nonlinear term) and puts them back after each candidate.
 extern "C" int32_t scl_ens2d_tune(scl_ens2d* e, int32_t reps, double* best_ms) {
   if (!e) return set_error("null argument"), SCL_EINVAL;
   if (reps < 1) return set_error("reps must be >= 1"), SCL_EINVAL;
   SCL_HIP(hipStreamSynchronize(e->stream));
-  for (int ug = 0; ug < (int)e->sizes.size(); ++ug) {
+  const size_t es = e->esize();
+  size_t save_bytes = 0;
+  for (const Group& gr : e->groups) save_bytes = std::max(save_bytes, 2 * gr.fs() * es);
+  void* save = nullptr;
+  if (hipMalloc(&save, save_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error("tuner: device allocation of %zu bytes failed", save_bytes), SCL_ENOMEM;
+  }
+  // fields 0: state, 1: previous nonlinear term (scheme 1)
+  auto copy = [&](const Group& gr, bool to_save) -> int32_t {
+    const size_t fb = gr.fs() * es;
+    char* b = static_cast<char*>(gr.base);
+    char* sv = static_cast<char*>(save);
+    const int fields[2] = {F_WS, F_ACC};
+    for (int k = 0; k < (e->p.scheme == 1 ? 2 : 1); ++k) {
+      char* dev = b + fields[k] * fb;
+      SCL_HIP(hipMemcpyAsync(to_save ? sv + k * fb : dev, to_save ? dev : sv + k * fb, fb, hipMemcpyDeviceToDevice,
+                             e->stream));
+    }
+    return SCL_OK;
+  };
+  const int64_t first = e->p.scheme == 1 ? std::max<int64_t>(e->steps, 2) : 0;
+  int32_t st = SCL_OK;
+  for (int ug = 0; ug < (int)e->sizes.size() && st == SCL_OK; ++ug) {
     const int n = e->sizes[ug];
     const int ng = e->p.per_member ? e->members[ug] : 1;
     double best = 1e300;
     int be = -1, bc = -1, br = -1;
     for (int ee : {8, 16})
-      for (int cw : {8, 16})
+      for (int cw : {4, 8, 16})
         for (int rg : {0, 1}) {
-          if (apply_schedule(e, ug, ee, cw, rg) != SCL_OK) continue;
-          // time the group's RHS (mode 0 writes only work fields)
-          auto rhs_all = [&]() -> int32_t {
-            for (int c = 0; c < ng; ++c) {
-              const Group& gr = e->groups[e->first[ug] + c];
-              SCL_CHECK(e->prec == SCL_SP ? stage<float>(*e, gr, 0, e->stream) : stage<double>(*e, gr, 0, e->stream));
-            }
-            return SCL_OK;
-          };
-          SCL_CHECK(rhs_all());
-          SCL_HIP(hipEventRecord(e->t0, e->stream));
-          for (int r = 0; r < reps; ++r) SCL_CHECK(rhs_all());
-          SCL_HIP(hipEventRecord(e->t1, e->stream));
-          SCL_HIP(hipEventSynchronize(e->t1));
-          float t = 0;
-          SCL_HIP(hipEventElapsedTime(&t, e->t0, e->t1));
-          const double per = (double)t / reps;
-          if (per < best) {
+          if (st != SCL_OK || apply_schedule(e, ug, ee, cw, rg) != SCL_OK) continue;
+          double per = 0.0;
+          for (int c = 0; c < ng && st == SCL_OK; ++c) {
+            const Group& gr = e->groups[e->first[ug] + c];
+            auto steps = [&](int k) -> int32_t {
+              return e->prec == SCL_SP ? group_steps<float>(*e, gr, e->stream, k, first)
+                                       : group_steps<double>(*e, gr, e->stream, k, first);
+            };
+            float t = 0;
+            auto timed = [&]() -> int32_t {
+              SCL_CHECK(copy(gr, true));
+              SCL_CHECK(steps(1));  // warm-up
+              SCL_HIP(hipEventRecord(e->t0, e->stream));
+              SCL_CHECK(steps(reps));
+              SCL_HIP(hipEventRecord(e->t1, e->stream));
+              SCL_CHECK(copy(gr, false));
+              SCL_HIP(hipEventSynchronize(e->t1));
+              SCL_HIP(hipEventElapsedTime(&t, e->t0, e->t1));
+              return SCL_OK;
+            };
+            st = timed();
+            per += (double)t / reps;
+          }
+          if (st == SCL_OK && per < best) {
             best = per;
             be = ee;
             bc = cw;
             br = rg;
           }
         }
-    if (be < 0) return set_error("no valid schedule for size %d", n), SCL_EINVAL;
-    SCL_CHECK(apply_schedule(e, ug, be, bc, br));
+    if (st != SCL_OK) break;
+    if (be < 0) {
+      st = (set_error("no valid schedule for size %d", n), SCL_EINVAL);
+      break;
+    }
+    st = apply_schedule(e, ug, be, bc, br);
     if (best_ms) best_ms[ug] = best;
   }
-  return SCL_OK;
+  (void)hipStreamSynchronize(e->stream);
+  (void)hipFree(save);
+  return st;
 }

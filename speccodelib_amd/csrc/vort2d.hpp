@@ -30,8 +30,10 @@ struct Col2D {
   static constexpr int T = N / E;
   static constexpr int BS = CW * T;
   static constexpr size_t lds_bytes = (size_t)(lds_pad(N) * CW + N) * sizeof(cpx<R>);
-  // radix schedules exist for E = 8 up to N = 512 and E = 16 up to 1024
-  static constexpr bool valid = (E == 8 ? N <= 512 : (E == 16 && N >= 16 && N <= 1024)) && BS <= 1024 &&
+  // radix schedules exist for E = 8 and E = 16 up to N = 1024 (at 1024
+  // points and E = 8, 8 columns make one 1024-thread workgroup: 4 waves
+  // per SIMD where E = 16 runs 2)
+  static constexpr bool valid = (E == 8 ? N <= 1024 : (E == 16 && N >= 16 && N <= 1024)) && BS <= 1024 &&
                                 BS >= 1 && lds_bytes <= 160 * 1024 && CW <= N / 2;
 };
 
