@@ -2,6 +2,7 @@
 // storage mapping, the IF-RK4 step as one hipGraph, the schedule autotuner.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <vector>
 
@@ -125,6 +126,9 @@ struct scl_ens2d {
   hipGraph_t graph[2] = {nullptr, nullptr};      // one step; a chunk of steps
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   int chunk = 10;  // steps per chunk graph (batched mode; 1 per member)
+  // One stream for all groups: 2.03 vs 2.20 ms/step for a graph branch per
+  // size group (SCL_ENS2D_FORK=1), whose concurrent groups slow each other.
+  bool fork_groups = false;
   double* diag = nullptr;
   int ncu = 256;
   int64_t steps = 0;
@@ -283,13 +287,14 @@ int32_t group_steps(const scl_ens2d& e, const Group& gr, hipStream_t st, int nst
   return SCL_OK;
 }
 
-// nsteps steps of every group.  Batched: the groups are independent, so each
-// runs its nsteps steps as one chain on its own branch (fork/join through
-// events, which graph capture turns into graph edges) and the branches
-// join once, at the end.
+// nsteps steps of every group, each group's nsteps steps as one chain, the
+// groups one after the other on the ensemble's stream.  With fork_groups
+// (batched mode) each group runs on its own branch instead (fork/join
+// through events, which graph capture turns into graph edges) and the
+// branches join once, at the end.
 // first: the index of the first step (scheme 1 bootstraps on steps 0, 1).
 int32_t launch_steps(scl_ens2d& e, int nsteps, int64_t first) {
-  const bool fork = !e.p.per_member && e.groups.size() > 1;
+  const bool fork = e.fork_groups && !e.p.per_member && e.groups.size() > 1;
   if (fork) SCL_HIP(hipEventRecord(e.fork, e.stream));
   for (auto& gr : e.groups) {
     hipStream_t st = fork ? gr.stream : e.stream;
@@ -555,6 +560,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
   // per-member mode keeps one step per graph: a chunk would hold
   // chunk x members x 12 nodes
   e->chunk = params->per_member ? 1 : 10;
+  if (const char* f = std::getenv("SCL_ENS2D_FORK")) e->fork_groups = std::atoi(f) != 0;
   auto fail = [&](int32_t st) {
     scl_ens2d_destroy(e);
     return st;

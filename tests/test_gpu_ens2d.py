@@ -92,14 +92,17 @@ def test_steps_fp64(per_member):
     e.close()
 
 
-def test_fp32_batched_per_member_eager_bitwise():
+def test_fp32_batched_per_member_eager_bitwise(monkeypatch):
     groups = [(64, 3), (128, 2)]
-    print("case ens2d fp32 batched/per-member/eager bitwise", flush=True)
+    print("case ens2d fp32 batched/per-member/eager/forked bitwise", flush=True)
     runs = []
-    for per_member, use_graph in ((False, True), (False, False), (True, True), (True, False)):
+    for per_member, use_graph, fork in ((False, True, 0), (False, False, 0), (True, True, 0), (True, False, 0),
+                                        (False, True, 1), (False, False, 1)):
+        monkeypatch.setenv("SCL_ENS2D_FORK", str(fork))  # a graph branch per size group
         e = scl.Ensemble2D(groups, prec=scl.SP, per_member=per_member, use_graph=use_graph, **KW)
         e.initialize(seed0=-31)
         e.step(4)
+        e.step(11)
         runs.append(_members(e, groups))
         e.close()
     for r in runs[1:]:
