@@ -281,6 +281,83 @@ def bench_les256(scl, steps, n=256):
     }
 
 
+def bench_ab2(scl, steps, n=512):
+    """The production drivers' IF-AB2 stepper (spectral_dns_dealiased.f90: CFL
+    dt, umax refreshed every 10 steps) on the 512^3 FP32 forced DNS: one RHS
+    per step on the same passes, the steady step as a graph."""
+    plan = scl.Plan3D(n, n, n, scl.SP)
+    d = scl.DNS(plan, nu=0.01, forced=1, fkmax=2.0, eps=0.1, stepper=1, beta=0.5, dtw=0.1, ieout=10,
+                umax=3.0 * np.sqrt(3.0), kcut=n // 2 - 1.0, use_graph=True)
+    d.initialize(1, 4.0, 1.0)
+    d.step(4)
+    d.sync()
+    t0 = time.perf_counter()
+    d.step(steps)
+    d.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    t, dt_last, k = d.step_info()
+    dg = d.diagnostics()
+    d.close()
+    plan.close()
+    print(f"[bench] IF-AB2 {n}^3: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+    return {
+        "workload": f"IF-AB2 (CFL dt, beta 0.5, umax every 10 steps) forced DNS {n}^3 FP32, nu=0.01",
+        "steps": steps,
+        "ms_per_step": ms,
+        "steps_per_s": 1e3 / ms,
+        "time": t,
+        "dt_last": dt_last,
+        "energy": dg["energy"],
+    }
+
+
+# turb_2d.f90's scheme on the configs[2] groups: per step an unshifted and a
+# shifted half RHS, 28 F2 (y-inverse 1 + 3 twice, x-pass 3 + 2 twice,
+# y-forward 2 + 1 then 2 + 1 + state and owz in and out).
+TURB2D_STEP_F2 = 28
+
+
+def bench_turb2d(scl, steps):
+    f2 = sum(m * 8 * (n // 2 + 1) * n for n, m in ENS2D_GROUPS)
+    e = scl.Ensemble2D(ENS2D_GROUPS, prec=scl.SP, dt=1e-4, scheme=1, sigma=1.0 / 400.0, nu=0.5, eta=5.0, kf=4.0)
+    e.initialize(seed0=-1, k0=8.0, energy=0.5)
+    e.tune(reps=3)
+    e.step(4)
+    ms = e.step_timed(steps) / steps
+    d = e.diagnostics()
+    e.close()
+    print(f"[bench] turb_2d scheme: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    return {
+        "workload": "turb_2d.f90 step (phase-shift dealiasing, (k sigma)^16 + (k/nu)^-4, band forcing, IF-AB2) on "
+                    "192x256^2 + 48x512^2 + 16x1024^2 FP32, dt=1e-4, sigma=1/400, nu=0.5, eta=5, kf=4",
+        "steps": steps,
+        "ms_per_step": ms,
+        "steps_per_s": 1e3 / ms,
+        "step_alg_bytes": TURB2D_STEP_F2 * f2,
+        "hbm_roofline_frac_step": TURB2D_STEP_F2 * f2 / (ms * 1e-3) / HBM_PEAK,
+        "finite": bool(np.all(np.isfinite(d))),
+    }
+
+
+def bench_padded(scl, reps, n=256):
+    """The 3/2-rule padded Lamb vector (scl_convec_padded) at 256^3 FP32 on
+    host arrays: six fields in, three out across PCIe per call, the products
+    on a 512^3 plan."""
+    plan = scl.Plan3D(n, n, n, scl.SP)
+    rng = np.random.default_rng(0)
+    f = [(rng.standard_normal(plan.shape) + 1j * rng.standard_normal(plan.shape)).astype(np.complex64)
+         for _ in range(6)]
+    plan.convec_padded(*[a.copy() for a in f])  # creates the padded plan
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.convec_padded(*f)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    plan.close()
+    print(f"[bench] padded convec {n}^3: {ms:.1f} ms/call", file=sys.stderr, flush=True)
+    return {"workload": f"scl_convec_padded {n}^3 FP32 (2n padded grid), host arrays", "calls": reps,
+            "ms_per_call": ms}
+
+
 def bench_stream(scl, nbytes=2 << 30, reps=10):
     """The HBM rate plain streaming kernels reach (the practical ceiling of
     the passes): read, write and copy of 2 GiB buffers."""
@@ -396,6 +473,10 @@ def main():
                     help="also time this many steps of the 1024^3 helical DNS (0: skip)")
     ap.add_argument("--les256", type=int, default=50,
                     help="also time this many steps of the 256^3 LES + scalar (0: skip)")
+    ap.add_argument("--ab2", type=int, default=20, help="also time this many IF-AB2 steps at 512^3 (0: skip)")
+    ap.add_argument("--turb2d", type=int, default=200,
+                    help="also time this many steps of the turb_2d scheme on the 2D groups (0: skip)")
+    ap.add_argument("--padded", type=int, default=3, help="also time this many padded convec calls (0: skip)")
     ap.add_argument("--no-main", action="store_true", help="skip the 512^3 DNS (profiling runs of the others)")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming HBM probe")
     ap.add_argument("--eager", action="store_true", help="launch the kernels directly instead of as graphs")
@@ -426,6 +507,12 @@ def main():
         out["heli1024"] = bench_heli1024(scl, a.heli1024)
     if a.les256:
         out["les256"] = bench_les256(scl, a.les256)
+    if a.ab2:
+        out["ab2_512"] = bench_ab2(scl, a.ab2)
+    if a.turb2d:
+        out["turb2d"] = bench_turb2d(scl, a.turb2d)
+    if a.padded:
+        out["padded256"] = bench_padded(scl, a.padded)
     if not a.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, scl)
