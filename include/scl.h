@@ -204,7 +204,9 @@ typedef struct scl_dns_params {
   int32_t scalar;      /* 1: advance phi with -div(u phi) - grad * u_z, diffusivity nu/schmidt */
   double grad;         /* mean scalar gradient along z */
   double schmidt;      /* Sc (Prandtl number pr of the reference) */
-  /* time stepper (DNS without LES or scalar for steppers 1 and 2) */
+  /* time stepper (DNS, LES and scalar alike: steppers 1 and 2 are also the
+   * loops of spectral_les.f90:241-304 and spectral_dns_scalar.f90:257-338,
+   * the scalar with the factors exp(-nu/Sc k^2 dt)) */
   int32_t stepper;     /* 0 Lawson IF-RK4 (dns-deform.f90:160-237);
                           1 IF-AB2 with CFL steps (spectral_dns_dealiased.f90:124-296):
                             an Euler step, a leapfrog step, then variable-step

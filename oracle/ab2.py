@@ -78,12 +78,18 @@ def ab2_update(istep, v, n, aux, dt, dto, E_dt, E_dto):
 
 
 class AB2:
-    """The IF-AB2 loop of spectral_dns_dealiased.f90 on three velocity fields."""
+    """The IF-AB2 loop of spectral_dns_dealiased.f90 on three velocity fields,
+    or, with rhs (a function of the fields returning their RHS) and nus (one
+    viscosity per field), the same loop of spectral_les.f90:241-304 and
+    spectral_dns_scalar.f90:257-338 (velocity and scalar, the scalar with
+    nu/Sc); umax comes from the first three fields."""
 
-    def __init__(self, p: AB2Params, v0):
+    def __init__(self, p: AB2Params, v0, rhs=None, nus=None):
         self.p = p
         n = p.nx
         self.dns = s3.DNS(s3.DNSParams(nx=n, nu=p.nu, dt=0.0, forced=p.forced, fkmax=p.fkmax, eps=p.eps))
+        self.rhs = rhs or self.dns.rhs
+        self.nus = list(nus) if nus is not None else [p.nu] * len(v0)
         self.k2 = self.dns.k2.astype(np.float64)
         kcut = p.kcut if p.kcut is not None else p.lx - 1.0
         self.kcut2 = kcut * kcut
@@ -115,16 +121,16 @@ class AB2:
         self.istep += 1
         if p.dt_fixed <= 0:
             if p.ieout > 0 and self.istep % p.ieout == 0:
-                self.umax = 3.5 * np.sqrt(2.0 * s3.energy(*self.v))
+                self.umax = 3.5 * np.sqrt(2.0 * s3.energy(*self.v[:3]))
             if p.dtw > 0 and abs(self.time - self.twrite) <= p.myeps:
                 self.twrite += p.dtw
-        n = self.dns.rhs(*self.v)
+        n = self.rhs(*self.v)
         dt = self._dt()
-        E_dt = np.exp(-p.nu * self.k2 * dt)
-        E_dto = np.exp(-p.nu * self.k2 * self.dto) if self.istep > 0 else None
         new = []
         aux = []
-        for q in range(3):
+        for q in range(len(self.v)):
+            E_dt = np.exp(-self.nus[q] * self.k2 * dt)
+            E_dto = np.exp(-self.nus[q] * self.k2 * self.dto) if self.istep > 0 else None
             a = self.aux[q] if self.aux is not None else None
             vn, an, dto = ab2_update(self.istep, self.v[q], n[q], a, dt, self.dto, E_dt, E_dto)
             new.append(vn)

@@ -140,10 +140,11 @@ class LESScalar:
         self.g = g_test(self.k2, k_test)
         self.delta_test = p.delta_ratio * p.delta
         self.nrhs = 0
+        self.rhs_per_step = 4  # the dynamic update's cadence: every iupdate steps (IF-AB2: 1 RHS a step)
 
     def rhs(self, vx, vy, vz, phi):
         p = self.p
-        if p.dynamic and self.nrhs % (4 * p.iupdate) == 0:
+        if p.dynamic and self.nrhs % (self.rhs_per_step * p.iupdate) == 0:
             _, _, _, self.cs2 = dynsmag_update(vx, vy, vz, self.kx, self.ky, self.kz, self.g, p.delta,
                                                self.delta_test, self.n)
         self.nrhs += 1

@@ -405,9 +405,10 @@ class DNS:
                  grad=0.0, schmidt=1.0, stepper=0, beta=0.5, dtw=0.0, ieout=10, umax=3.0 * 3.0**0.5, dt_first=0.0,
                  kcut=0.0):
         """stepper: 0 IF-RK4 (dns-deform.f90), 1 IF-AB2 with CFL steps
-        (spectral_dns_dealiased.f90; beta, dtw, ieout, umax, dt_first), 2
-        IF-AB2 with fixed dt (specdns-forced-omp-ft.f90).  kcut: truncation
-        radius after each step (0: 2 lx/3)."""
+        (spectral_dns_dealiased.f90, and with sgs / scalar the loops of
+        spectral_les.f90 and spectral_dns_scalar.f90; beta, dtw, ieout, umax,
+        dt_first), 2 IF-AB2 with fixed dt (specdns-forced-omp-ft.f90).
+        kcut: truncation radius after each step (0: 2 lx/3)."""
         self.plan = plan
         self.params = ScDnsParams(nu, dt, int(forced), fkmax, eps, eta, int(dealias_rhs), int(use_graph), int(sgs), cs2,
                                   delta, delta_test, k_test, int(iupdate), int(scalar), grad, schmidt, int(stepper),

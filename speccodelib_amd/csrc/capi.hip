@@ -648,14 +648,10 @@ static int32_t rk4_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_par
 // fused with the z-inverse of the new v for the next step, then the kx = 0
 // Hermitian copy on v and the time update (k_ab_end).  istep: steps since
 // the state was set.
+// The step's dt and weights into ab (the umax energy first on refresh steps).
 template <class R>
-static int32_t ab2_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, double* ab,
-                            double* ab_part, int64_t istep, bool primed, Timer* tm) {
+static int32_t ab_begin(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, double* ab, double* ab_part) {
   const Geom& g = p->g;
-  RhsParams<R> rp = rhs_params<R>(p, prm.nu, 0.0, prm.forced, prm.fkmax, 0);
-  apply_kcut<R>(rp, prm);
-  rp.nonfinite = d.nonfinite;
-  rp.ab = ab;
   AbConsts c;
   c.stepper = prm.stepper;
   c.ieout = prm.ieout;
@@ -663,33 +659,50 @@ static int32_t ab2_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_par
   c.dtw = prm.dtw;
   c.myeps = 2.0 * (double)std::numeric_limits<float>::epsilon();
   c.dt_fixed = prm.dt;
-  Timer none;
-  if (!tm) tm = &none;
-  tm->mark(-1);
   if (prm.stepper == 1) {
     hipLaunchKernelGGL((k_ab_energy<R>), dim3(DIAG_BLOCKS), dim3(256), 0, p->stream, d.V<R>(), d.fs, g,
                        (const double*)ab, prm.ieout, ab_part);
     SCL_CHECK(check_launch("ab_energy"));
   }
   hipLaunchKernelGGL(k_ab_begin, dim3(1), dim3(64), 0, p->stream, ab, (const double*)ab_part, DIAG_BLOCKS, c);
-  SCL_CHECK(check_launch("ab_begin"));
-  const int mode = istep == 0 ? 5 : istep == 1 ? 6 : 7;
-  SCL_CHECK(rk_stage<R>(p, d, mode, rp, prm.eps, prm.eta, prm.fkmax, d.V<R>(), tm, primed, true));
+  return check_launch("ab_begin");
+}
+
+static int32_t ab_end(scl_plan3d* p, double* ab) {
   hipLaunchKernelGGL(k_ab_end, dim3(1), dim3(64), 0, p->stream, ab);
   return check_launch("ab_end");
 }
 
-// The LES and/or scalar RHS (multi-pass) for stage `mode`.
+template <class R>
+static int32_t ab2_step_dev(scl_plan3d* p, const DevFields& d, const scl_dns_params& prm, double* ab,
+                            double* ab_part, int64_t istep, bool primed, Timer* tm) {
+  RhsParams<R> rp = rhs_params<R>(p, prm.nu, 0.0, prm.forced, prm.fkmax, 0);
+  apply_kcut<R>(rp, prm);
+  rp.nonfinite = d.nonfinite;
+  rp.ab = ab;
+  Timer none;
+  if (!tm) tm = &none;
+  tm->mark(-1);
+  SCL_CHECK(ab_begin<R>(p, d, prm, ab, ab_part));
+  const int mode = istep == 0 ? 5 : istep == 1 ? 6 : 7;
+  SCL_CHECK(rk_stage<R>(p, d, mode, rp, prm.eps, prm.eta, prm.fkmax, d.V<R>(), tm, primed, true));
+  return ab_end(p, ab);
+}
+
+// The LES and/or scalar RHS (multi-pass) for stage `mode` (IF-RK4 1..4,
+// IF-AB2 5..7 with the weights in ab).
 template <class R>
 static int32_t rk_stage_ext(scl_plan3d* p, const DevFields& d, int mode, const scl_dns_params& prm, const cpx<R>* st,
-                            Timer* tm) {
+                            Timer* tm, const double* ab = nullptr) {
   const int64_t fs = d.fs;
   Timer none;
   if (!tm) tm = &none;
   ExtLayout L{prm.sgs ? 1 : 0, prm.scalar ? 1 : 0};
   ExtParams<R> ep;
   ep.r = rhs_params<R>(p, prm.nu, prm.dt, prm.forced, prm.fkmax, prm.dealias_rhs);
+  apply_kcut<R>(ep.r, prm);
   ep.r.nonfinite = d.nonfinite;
+  ep.r.ab = ab;
   ep.nu_phi = (R)(prm.nu / (prm.schmidt > 0 ? prm.schmidt : 1.0));
   ep.grad = (R)prm.grad;
   ep.sign_phi = 1;
@@ -855,8 +868,6 @@ extern "C" int32_t scl_dns_create(scl_plan3d* p, const scl_dns_params* prm, scl_
   if (prm->forced < 0 || prm->forced > 2) return set_error("forced must be 0, 1 or 2"), SCL_EINVAL;
   if (prm->sgs < 0 || prm->sgs > 2) return set_error("sgs must be 0, 1 or 2"), SCL_EINVAL;
   if (prm->stepper < 0 || prm->stepper > 2) return set_error("stepper must be 0, 1 or 2"), SCL_EINVAL;
-  if (prm->stepper && (prm->sgs || prm->scalar))
-    return set_error("the IF-AB2 steppers take the DNS without LES or scalar"), SCL_EINVAL;
   if (prm->stepper == 1 && !(prm->beta > 0 && prm->umax > 0))
     return set_error("stepper 1 needs beta > 0 and umax > 0"), SCL_EINVAL;
   if (prm->stepper == 2 && !(prm->dt > 0)) return set_error("stepper 2 needs dt > 0"), SCL_EINVAL;
@@ -967,6 +978,7 @@ extern "C" int32_t scl_dns_set_scalar(scl_dns* d, const void* phi) {
   if (!d || d->nc < 4) return set_error("solver has no scalar"), SCL_EINVAL;
   const void* vi[1] = {phi};
   SCL_CHECK(clear_nonfinite(d));
+  SCL_CHECK(ab_reset(d));
   return dns_io(d, 3, 1, vi, nullptr);
 }
 
@@ -1012,6 +1024,16 @@ static int32_t dns_step_once(scl_dns* d, Timer* tm) {
     const double dtest = prm.delta_test > 0 ? prm.delta_test : 2.0 * delta;
     SCL_CHECK(ExtOps<R>::dyn_update(*p, f.V<R>(), (cpx<R>*)d->dyn, f.fs, (const R*)d->gt, (R)delta, (R)dtest, d->coef,
                                     d->part));
+  }
+  if (prm.stepper) {
+    // IF-AB2 of spectral_les.f90 / spectral_dns_scalar.f90: one RHS of the
+    // LES / scalar passes with the AB2 combine
+    SCL_CHECK(ab_begin<R>(p, f, prm, d->ab, d->ab_part));
+    const int mode = d->ab_steps == 0 ? 5 : d->ab_steps == 1 ? 6 : 7;
+    SCL_CHECK(rk_stage_ext<R>(p, f, mode, prm, f.V<R>(), tm, d->ab));
+    SCL_CHECK(ab_end(p, d->ab));
+    d->ab_steps += 1;
+    return SCL_OK;
   }
   for (int stage = 1; stage <= 4; ++stage) {
     const cpx<R>* st = stage == 1 ? f.V<R>() : f.S<R>();

@@ -247,6 +247,27 @@ struct ExtParams {
   int sign_phi;  // +1: R = -i k.F (physical advection)
 };
 
+// The update of one component in the LES / scalar combine: the IF-RK4 stage
+// MODE 1..4 (stage_update), or the IF-AB2 step of MODE 5..7 (ab_keep, the
+// weights this step's k_ab_begin left in r.ab) as spectral_les.f90:241-304
+// and spectral_dns_scalar.f90:257-338 write it, with the component's
+// viscosity nu (nu for u, nu/Sc for the scalar); AB2 also zeroes the mean.
+template <int MODE, class R, class I>
+__device__ inline void ext_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R nu, R k2, bool cut, bool mean,
+                                  const RhsParams<R>& r) {
+  if constexpr (MODE >= 5) {
+    const R dt = (R)r.ab[AB_DT];
+    const R xk = -nu * k2;
+    const R e1 = expm1_r(xk * dt);
+    const R e2 = MODE == 5 ? (R)0 : expm1_r(xk * (R)r.ab[AB_DTO]);
+    (void)ab_keep<MODE, R, I, false>(a, o, V, DU, S, dt, (R)r.ab[AB_S], (R)r.ab[AB_EK], (R)r.ab[AB_DELTA], e1, e2,
+                                      cut || mean, r.nonfinite);
+  } else {
+    const R x = -nu * k2 * r.dt / (R)2;
+    stage_update<MODE, R>(r.dt * a, o, V, DU, S, expm1_r(x), x, cut, r.nonfinite);
+  }
+}
+
 // Combine (spectral): N = u x w + f - i k_j tau_ij (spectral_les.f90:232-234),
 // symmetrize's zeroing, projection; R_phi = -i k . F - G u_z
 // (spectral_dns_scalar.f90:236,245), zeroing; then the IF-RK4 stage (MODE
@@ -317,16 +338,11 @@ __global__ __launch_bounds__(256) void k_ext_combine(cpx<R>* W, int64_t fs, cpx<
     W[i + 2 * fs] = d;
     if (L.scalar) W[i + 3 * fs] = rp;
   } else {
-    const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
-    const R em1 = expm1_r(xv);
-    const bool cut = k2 >= p.r.kc2;
-    stage_update<MODE, R>(p.r.dt * a, i, V, DU, S, em1, xv, cut, p.r.nonfinite);
-    stage_update<MODE, R>(p.r.dt * b, i + fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
-    stage_update<MODE, R>(p.r.dt * d, i + 2 * fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
-    if (L.scalar) {
-      const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
-      stage_update<MODE, R>(p.r.dt * rp, i + 3 * fs, V, DU, S, expm1_r(xp), xp, cut, p.r.nonfinite);
-    }
+    const bool cut = k2 >= p.r.kc2, mean = ik2 == 0;
+    ext_update<MODE, R>(a, i, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+    ext_update<MODE, R>(b, i + fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+    ext_update<MODE, R>(d, i + 2 * fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+    if (L.scalar) ext_update<MODE, R>(rp, i + 3 * fs, V, DU, S, p.nu_phi, k2, cut, mean, p.r);
   }
 }
 
@@ -679,16 +695,11 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zfwd(cpx<R>* W, int6
       W[o + 2 * fs] = cz;
       if (L.scalar) W[o + 3 * fs] = r;
     } else {
-      const R xv = -p.r.nu * k2 * p.r.dt / (R)2;
-      const R em1 = expm1_r(xv);
-      const bool cut = k2 >= p.r.kc2;
-      stage_update<MODE, R>(p.r.dt * ax, o, V, DU, S, em1, xv, cut, p.r.nonfinite);
-      stage_update<MODE, R>(p.r.dt * by, o + fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
-      stage_update<MODE, R>(p.r.dt * cz, o + 2 * fs, V, DU, S, em1, xv, cut, p.r.nonfinite);
-      if (L.scalar) {
-        const R xp = -p.nu_phi * k2 * p.r.dt / (R)2;
-        stage_update<MODE, R>(p.r.dt * r, o + 3 * fs, V, DU, S, expm1_r(xp), xp, cut, p.r.nonfinite);
-      }
+      const bool cut = k2 >= p.r.kc2, mean = ik2 == 0;
+      ext_update<MODE, R>(ax, o, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+      ext_update<MODE, R>(by, o + fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+      ext_update<MODE, R>(cz, o + 2 * fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+      if (L.scalar) ext_update<MODE, R>(r, o + 3 * fs, V, DU, S, p.nu_phi, k2, cut, mean, p.r);
     }
   }
 }

@@ -64,6 +64,9 @@ struct ExtOps {
       case 2: hipLaunchKernelGGL((k_ext_combine<R, 2>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
       case 3: hipLaunchKernelGGL((k_ext_combine<R, 3>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
       case 4: hipLaunchKernelGGL((k_ext_combine<R, 4>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
+      case 5: hipLaunchKernelGGL((k_ext_combine<R, 5>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
+      case 6: hipLaunchKernelGGL((k_ext_combine<R, 6>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
+      case 7: hipLaunchKernelGGL((k_ext_combine<R, 7>), grid_all(g), dim3(256), 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L); break;
       default: set_error("bad mode %d", mode); return SCL_EINVAL;
     }
     return check_launch("ext_combine");

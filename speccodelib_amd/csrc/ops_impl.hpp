@@ -313,6 +313,9 @@ int32_t Ops<R>::ext_zfwd(const scl_plan3d& pl, int mode, C* W, int64_t fs, C* V,
         case 2: hipLaunchKernelGGL((k_ext_zfwd<R, N, 2>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
         case 3: hipLaunchKernelGGL((k_ext_zfwd<R, N, 3>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
         case 4: hipLaunchKernelGGL((k_ext_zfwd<R, N, 4>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 5: hipLaunchKernelGGL((k_ext_zfwd<R, N, 5>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 6: hipLaunchKernelGGL((k_ext_zfwd<R, N, 6>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
+        case 7: hipLaunchKernelGGL((k_ext_zfwd<R, N, 7>), grd, blk, 0, pl.stream, W, fs, V, DU, S, st, coef, ep, g, L, tw, tws); break;
         default: set_error("bad mode %d", mode); return SCL_EINVAL;
       }
       return check_launch("ext_zfwd");

@@ -2,12 +2,14 @@
 combine passes forced at small sizes (SCL_EXT_UNFUSED=1): the path plans of
 1024-point z-lines take, where the fused z-passes of those RHS would spill.
 It runs the prep kernels, the z-passes of the plain column transform, the
-x-pass with the physical products, the separate combine kernels, and the
+x-pass with the physical products, the separate combine kernels (IF-RK4
+and IF-AB2), and the
 adjoint's prep, pointwise and r2c passes without the physical tape, against
 the same oracle bars."""
 import pytest
 
 import speccodelib_amd as scl
+from tests import test_gpu_ab2_les as ab2les
 from tests import test_gpu_adjoint as adj
 from tests import test_gpu_les as les
 
@@ -28,6 +30,11 @@ def test_les_rhs_and_solver_fp64():
     les.test_les_scalar_rhs_fp64()
     for dynamic in (False, True):
         les.test_les_scalar_solver_fp64(dynamic)
+
+
+def test_les_scalar_ab2_fp64():
+    for dynamic in (False, True):
+        ab2les.test_les_scalar_ab2_fp64(dynamic)
 
 
 @pytest.mark.parametrize("sgs", [0, 1])

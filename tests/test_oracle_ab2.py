@@ -46,6 +46,39 @@ def test_exact_viscous_decay_with_variable_steps():
     assert np.min(np.abs(t[:, None] - 0.05 * np.arange(1, 5)[None, :]), axis=0).max() < 1e-12
 
 
+def _check_per_field_viscosity(cls):
+    """With the RHS off, each field decays by its own exp(-nu_q k^2 t): the
+    velocity with nu, the scalar with nu/Sc (spectral_dns_scalar.f90:266-322)."""
+    n, nu, sc = 16, 0.05, 0.3
+    p = ab2.AB2Params(nx=n, nu=nu, forced=False, beta=0.3, dtw=0.05, ieout=2, umax=3.0)
+    rng = np.random.default_rng(4)
+    v0 = _abc(n) + [orc.r2c(rng.standard_normal((n, n, n))) / n**3]
+    for a in v0:
+        a[orc.wavenumber(n, n, n)[3] >= (n // 2 - 1.0) ** 2] = 0
+        a[0, 0, 0] = 0
+    s = cls(p, v0, rhs=lambda *v: [np.zeros_like(a) for a in v], nus=[nu, nu, nu, nu / sc])
+    for _ in range(12):
+        s.step()
+    k2 = orc.wavenumber(n, n, n)[3]
+    for q, a in enumerate(s.v):
+        want = v0[q] * np.exp(-(nu if q < 3 else nu / sc) * k2 * s.time)
+        assert np.abs(a - want).max() < 1e-13 * np.abs(v0[q]).max(), q
+
+
+def test_per_field_viscosity():
+    _check_per_field_viscosity(ab2.AB2)
+
+
+def test_per_field_viscosity_check_catches_shared_viscosity():
+    src = inspect.getsource(ab2.AB2)
+    old = "E_dt = np.exp(-self.nus[q] * self.k2 * dt)"
+    assert old in src
+    ns = dict(ab2.__dict__)
+    exec(compile(src.replace(old, "E_dt = np.exp(-self.p.nu * self.k2 * dt)"), "<mutant>", "exec"), ns)
+    with pytest.raises(AssertionError):
+        _check_per_field_viscosity(ns["AB2"])
+
+
 # --------------------------------------------------------------------------
 def _scalar_run(update, lam, dts):
     """y' = -lam y + cos(t), y(0) = 1, through the stepper's update formulas
