@@ -146,6 +146,35 @@ module scl_capi
       integer(c_int32_t), value :: lx1, ly, lz, nx, ny, nz
     end function
 
+    integer(c_int32_t) function scl_padd(prec, u, ub, lx1, ly, lz, lxb1, lyb, lzb) bind(C, name='scl_padd')
+      import
+      integer(c_int32_t), value :: prec, lx1, ly, lz, lxb1, lyb, lzb
+      type(*), dimension(*) :: u, ub
+    end function
+
+    integer(c_int32_t) function scl_unpadd(prec, ub, u, lx1, ly, lz, lxb1, lyb, lzb) bind(C, name='scl_unpadd')
+      import
+      integer(c_int32_t), value :: prec, lx1, ly, lz, lxb1, lyb, lzb
+      type(*), dimension(*) :: ub, u
+    end function
+
+    integer(c_int32_t) function scl_convec_padded(plan, vx, vy, vz, wx, wy, wz, lx1, ly, lz, lxb1, lyb, lzb, &
+        nxb, nyb, nzb) bind(C, name='scl_convec_padded')
+      import
+      type(c_ptr), value :: plan
+      type(*), dimension(*) :: vx, vy, vz, wx, wy, wz
+      integer(c_int32_t), value :: lx1, ly, lz, lxb1, lyb, lzb, nxb, nyb, nzb
+    end function
+
+    integer(c_int32_t) function scl_smag_dealias(plan, vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz, &
+        lx1, ly, lz, lxb1, lyb, lzb, nxb, nyb, nzb, delta) bind(C, name='scl_smag_dealias')
+      import
+      type(c_ptr), value :: plan
+      type(*), dimension(*) :: vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz
+      integer(c_int32_t), value :: lx1, ly, lz, lxb1, lyb, lzb, nxb, nyb, nzb
+      real(c_double), value :: delta
+    end function
+
     integer(c_int32_t) function scl_skewness(plan, ux, kx, s, nx, ny, nz) bind(C, name='scl_skewness')
       import
       type(c_ptr), value :: plan

@@ -222,23 +222,6 @@ subroutine rk4step(vx, vy, vz, lx1, ly, lz, rnu, dt, iforce, fkmax, eps, eta)
                              real(fkmax, c_double), real(eps, c_double), real(eta, c_double)), 'rk4step')
 end subroutine rk4step
 
-! smag.f90:1-78 (c1 = 0.026 inside the library)
-subroutine smag(vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz, lx1, ly, lz, nx, ny, nz, delta)
-  use iso_c_binding
-  use mconstant
-  use scl_capi
-  implicit none
-  integer, intent(in) :: lx1, ly, lz, nx, ny, nz
-  real(sp), intent(in) :: delta
-  complex(sp), dimension(lx1, ly, lz), intent(in) :: vx, vy, vz
-  complex(sp), dimension(lx1, ly, lz), intent(out) :: t11, t12, t13, t22, t23, t33
-  real(sp), dimension(lx1), intent(in) :: kx
-  real(sp), dimension(ly), intent(in) :: ky
-  real(sp), dimension(lz), intent(in) :: kz
-  call scl_check(scl_smag(c_null_ptr, vx, vy, vz, t11, t12, t13, t22, t23, t33, kx, ky, kz, lx1, ly, lz, &
-                          nx, ny, nz, real(delta, c_double)), 'smag')
-end subroutine smag
-
 ! dynsmag.f90:1-236.  The logical update and the real(sp) mm, lm, cs2 of the
 ! reference become the C ABI's int32 and doubles here; with update false
 ! cs2 is read (the static branch, :196-233) and mm, lm are left as they are.

@@ -105,3 +105,38 @@ def test_fortran_derivs_driver_matches_oracle(tmp_path):
     err = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(got, y))
     print(f"derivs RK4 rel err {err:.2e}", flush=True)
     assert err < 1e-10
+
+
+PADDED_EXE = os.path.join(ROOT, "speccodelib_amd", "fortran", "_build", "padded_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(PADDED_EXE), reason="Fortran driver not built (amdflang)")
+def test_fortran_padded_routines(tmp_path):
+    """padd, unpadd, the padded convec and smag_dealias.f90's smag called by
+    their reference names from Fortran (libscl_fortran_padded.a linked before
+    libscl_fortran.a), FP32, against oracle/padded.py on the 2n grid the
+    library pads to (and padd/unpadd exactly)."""
+    from oracle import padded as pd
+
+    n = 32
+    out = tmp_path / "padded.bin"
+    print("case fortran padded routines", flush=True)
+    r = subprocess.run([PADDED_EXE, str(n), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lx1, nb = n // 2 + 1, 3 * n // 2
+    a = np.fromfile(out, np.complex64)
+    m = n * n * lx1
+    f = a[:16 * m].reshape(16, n, n, lx1)
+    ub = a[16 * m:].reshape(nb, nb, nb // 2 + 1)
+    v, w, lamb, tau, u2 = f[0:3], f[3:6], f[6:9], f[9:15], f[15]
+    np.testing.assert_array_equal(ub, pd.padd(v[0], nb // 2 + 1, nb, nb))
+    np.testing.assert_array_equal(u2, v[0])
+    vd = [x.astype(np.complex128) for x in v]
+    want = pd.convec_padded(*vd, *[x.astype(np.complex128) for x in w], (n, n, n), nb=(2 * n,) * 3)
+    e1 = max(np.abs(x - y).max() for x, y in zip(lamb, want)) / max(np.abs(y).max() for y in want)
+    kx, ky, kz, _ = orc.wavenumber(n, n, n)
+    tw = pd.smag_dealias(*vd, kx, ky, kz, (n, n, n), float(np.float32(2 * np.float32(3.1415926536) / n)),
+                         nb=(2 * n,) * 3)
+    e2 = max(np.abs(x - y).max() for x, y in zip(tau, tw)) / max(np.abs(y).max() for y in tw)
+    print(f"padded convec {e1:.2e}, smag_dealias {e2:.2e}", flush=True)
+    assert e1 < 2e-6 and e2 < 2e-6
