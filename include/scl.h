@@ -172,6 +172,14 @@ int32_t scl_smag_dealias(scl_plan3d *plan, const void *vx, const void *vy, const
  * d = du_x/dx in physical space, summed in FP64.  kx must be wavenumber()'s. */
 int32_t scl_skewness(scl_plan3d *plan, const void *ux, const void *kx, double *s, int32_t nx, int32_t ny, int32_t nz);
 
+/* Helical initial field (initialize_heli.f90:1-109): per mode two complex
+ * Gaussians (am, then ap) from ran1 pairs in (z, y, x) order; am rescaled
+ * per shell so that sum |am|^2 = (1 - hrel) sum |ap|^2 (the file's shell
+ * masks); u = sqrt(3/2) (ap h+ + am h-) u0 sqrt(8 sqrt(2/pi)/(3 pi akp^5))
+ * k exp(-k^2/akp^2) with the helical basis i k x h+ = |k| h+; symmetrize;
+ * sum' |u|^2 = 1.5 u0^2.  Host arrays in the reference layout. */
+int32_t scl_initialize_heli(int32_t prec, void *ux, void *uy, void *uz, int32_t nx, int32_t ny, int32_t nz,
+                            int32_t iseed, double hrel, double akp, double u0);
 /* initialize(vx,vy,vz,kx,ky,kz,k2,iseed,lx1,ly,lz,akp,u0,kcut)
  * (initialize.f90:373-463 draw order, :328-332 complex projection), host
  * only, in the real type of prec.  kcut <= 0 selects 2*lx/3. */
@@ -247,6 +255,8 @@ int32_t scl_dns_set_scalar(scl_dns *d, const void *phi);
 int32_t scl_dns_get_scalar(scl_dns *d, void *phi);
 int32_t scl_dns_get_velocity(scl_dns *d, void *vx, void *vy, void *vz);
 int32_t scl_dns_initialize(scl_dns *d, int32_t iseed, double akp, double u0);
+/* The solver's state from scl_initialize_heli (spectral_heli_dns.f90:92). */
+int32_t scl_dns_initialize_heli(scl_dns *d, int32_t iseed, double hrel, double akp, double u0);
 /* nsteps IF-RK4 steps, asynchronous on the plan's stream */
 int32_t scl_dns_step(scl_dns *d, int32_t nsteps);
 int32_t scl_dns_sync(scl_dns *d);

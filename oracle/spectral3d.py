@@ -511,3 +511,48 @@ def diagnostics(vx, vy, vz, dns: DNS) -> Diag:
         divmax=divergence_max(vx, vy, vz, dns.kx, dns.ky, dns.kz, p.nx),
         spectrum=spectrum(vx, vy, vz, dns.k2),
     )
+
+
+def initialize_heli(nx, ny, nz, iseed, hrel, akp, u0):
+    """initialize_heli.f90:1-109 in FP64: per mode (x fastest) two complex
+    Gaussians am then ap from ran1 pairs (:22-35); am rescaled on the shells
+    |k - ii| < 1/2 by sqrt((1 - hrel) P/M), P and M the sums of |ap|^2 and
+    |am|^2 over |k - ii - oneless/2| < 1/2 (:37-44, the two masks as
+    written); u = sqrt(3/2) (ap h+ + am conj(h+)) A(k) with the basis of
+    :46-88 and A(k) = u0 sqrt(8 sqrt(2/pi)/(3 pi akp^5)) k exp(-k^2/akp^2);
+    the mean zero (the file's basis is 0/0 there); symmetrize; sum' |u|^2 =
+    1.5 u0^2 (:90-101).  Pinned by tests/test_oracle_ic.py."""
+    kx, ky, kz, k2 = wavenumber(nx, ny, nz)
+    lx1 = nx // 2 + 1
+    nm = nz * ny * lx1
+    d = ran1_stream(iseed, 4 * nm).astype(np.float64).reshape(nm, 4)
+    t = np.minimum(d / 2147483647.0, 1.0 - 1.2e-7)
+    g = np.sqrt(-2.0 * np.log(t[:, [0, 2]])) * np.exp(1j * 2.0 * PI_REF * t[:, [1, 3]]) / np.sqrt(2.0)
+    am = g[:, 0].reshape(nz, ny, lx1).copy()
+    ap = g[:, 1].reshape(nz, ny, lx1).copy()
+    kk = np.sqrt(np.round(k2).astype(np.float64))
+    kk[0, 0, 0] = 0.0
+    oneless = 1.0 - 2.0 * float(np.finfo(np.float32).eps)
+    for ii in range(1, lx1):
+        m = np.abs(kk - ii - 0.5 * oneless) < 0.5
+        P = np.sum(np.abs(ap[m]) ** 2)
+        M = np.sum(np.abs(am[m]) ** 2)
+        am[np.abs(kk - ii) < 0.5] *= np.sqrt((1.0 - hrel) * P / M) if M > 0 else 0.0
+    KX = np.broadcast_to(kx[None, None, :], kk.shape).astype(np.float64)
+    KY = np.broadcast_to(ky[None, :, None], kk.shape).astype(np.float64)
+    KZ = np.broadcast_to(kz[:, None, None], kk.shape).astype(np.float64)
+    h = np.sqrt(2.0) / 2.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        axis = (KY == 0) & (KZ == 0)
+        k2d = np.where(axis, np.maximum(np.sqrt(KX**2 + KY**2), 2.0 * float(np.finfo(np.float32).eps)),
+                       np.sqrt(KZ**2 + KY**2))
+        hpx = np.where(axis, h * (-KY / k2d - 1j * KX * KZ / kk / k2d), h * 1j * (KY**2 + KZ**2) / kk / k2d)
+        hpy = np.where(axis, h * (KX / k2d - 1j * KY * KZ / kk / k2d), h * (-KZ / k2d - 1j * KX * KY / kk / k2d))
+        hpz = np.where(axis, h * 1j * (KY**2 + KX**2) / kk / k2d, h * (KY / k2d - 1j * KZ * KX / kk / k2d))
+    amp = u0 * np.sqrt(8.0 * np.sqrt(2.0 / PI_REF) / (3.0 * PI_REF * akp**5)) * kk * np.exp(-kk * kk / (akp * akp))
+    u = [np.sqrt(1.5) * (ap * hp + am * np.conj(hp)) * amp for hp in (hpx, hpy, hpz)]
+    for a in u:
+        a[0, 0, 0] = 0
+        symmetrize3d(a)
+    s = np.sqrt(1.5 * u0 * u0 / energy(*u))
+    return [a * s for a in u]

@@ -143,6 +143,8 @@ def lib():
             "scl_projection": ([P] + [P] * 6 + [I32] * 3, I32),
             "scl_ratesdudt": ([P] + [P] * 6 + [I32, D, D, D], I32),
             "scl_initialize3d": ([I32, P, P, P, I32, I32, I32, I32, D, D, D], I32),
+            "scl_initialize_heli": ([I32, P, P, P, I32, I32, I32, I32, D, D, D], I32),
+            "scl_dns_initialize_heli": ([P, I32, D, D, D], I32),
             "scl_rk4step": ([P, P, P, P, D, D, I32, D, D, D], I32),
             "scl_dns_create": ([P, ctypes.POINTER(ScDnsParams), ctypes.POINTER(P)], I32),
             "scl_dns_destroy": ([P], I32),
@@ -397,6 +399,17 @@ def initialize(nx, ny=None, nz=None, iseed=1, akp=4.0, u0=1.0, prec=SP, kcut=-1.
     return tuple(out)
 
 
+def initialize_heli(nx, ny=None, nz=None, iseed=1, hrel=1.0, akp=4.0, u0=1.0, prec=SP):
+    """initialize_heli(ux,uy,uz,kx,ky,kz,k2,lx1,ly,lz,iseed,hrel,akp,u0)
+    (initialize_heli.f90:1-109): a helical field of relative helicity set by
+    hrel (1: every mode of the rescaled shells positively helical)."""
+    ny = ny or nx
+    nz = nz or nx
+    out = [np.zeros((nz, ny, nx // 2 + 1), _cdtype(prec)) for _ in range(3)]
+    _check(lib().scl_initialize_heli(prec, *map(_ptr, out), nx, ny, nz, iseed, hrel, akp, u0))
+    return tuple(out)
+
+
 class DNS:
     """Device-resident IF-RK4 solver: the dns-deform.f90 loop body."""
 
@@ -445,6 +458,10 @@ class DNS:
 
     def initialize(self, iseed=1, akp=4.0, u0=1.0):
         _check(lib().scl_dns_initialize(self.h, iseed, akp, u0))
+
+    def initialize_heli(self, iseed=1, hrel=1.0, akp=4.0, u0=1.0):
+        """The state from initialize_heli (spectral_heli_dns.f90:92)."""
+        _check(lib().scl_dns_initialize_heli(self.h, iseed, hrel, akp, u0))
 
     def step(self, n=1):
         _check(lib().scl_dns_step(self.h, n))

@@ -794,6 +794,14 @@ extern "C" int32_t scl_rk4step(scl_plan3d* p, void* vx, void* vy, void* vz, doub
   });
 }
 
+extern "C" int32_t scl_initialize_heli(int32_t prec, void* ux, void* uy, void* uz, int32_t nx, int32_t ny, int32_t nz,
+                                       int32_t iseed, double hrel, double akp, double u0) {
+  if (!ux || !uy || !uz || nx < 4 || ny < 2 || nz < 2 || (nx & 1) || (ny & 1) || (nz & 1) || !(hrel >= 0 && hrel <= 1))
+    return set_error("bad initialize_heli arguments (even sizes, 0 <= hrel <= 1)"), SCL_EINVAL;
+  if (prec == SCL_SP) return initialize_heli3d<float>((float*)ux, (float*)uy, (float*)uz, nx, ny, nz, iseed, hrel, akp, u0);
+  return initialize_heli3d<double>((double*)ux, (double*)uy, (double*)uz, nx, ny, nz, iseed, hrel, akp, u0);
+}
+
 extern "C" int32_t scl_initialize3d(int32_t prec, void* vx, void* vy, void* vz, int32_t nx, int32_t ny, int32_t nz,
                                     int32_t iseed, double akp, double u0, double kcut) {
   if (!vx || !vy || !vz || nx < 4 || ny < 2 || nz < 2 || (nx & 1)) return set_error("bad initialize arguments"), SCL_EINVAL;
@@ -999,6 +1007,16 @@ extern "C" int32_t scl_dns_initialize(scl_dns* d, int32_t iseed, double akp, dou
   }
   std::vector<double> a(n), b(n), c(n);
   SCL_CHECK(initialize3d<double>(a.data(), b.data(), c.data(), g.nx, g.ny, g.nz, iseed, akp, u0, -1.0));
+  return scl_dns_set_velocity(d, a.data(), b.data(), c.data());
+}
+
+extern "C" int32_t scl_dns_initialize_heli(scl_dns* d, int32_t iseed, double hrel, double akp, double u0) {
+  if (!d) return set_error("null solver"), SCL_EINVAL;
+  const Geom& g = d->plan->g;
+  const size_t n = (size_t)(g.mx + 1) * g.nrow * 2;
+  const size_t es = d->plan->prec == SCL_SP ? sizeof(float) : sizeof(double);
+  std::vector<char> a(n * es), b(n * es), c(n * es);
+  SCL_CHECK(scl_initialize_heli(d->plan->prec, a.data(), b.data(), c.data(), g.nx, g.ny, g.nz, iseed, hrel, akp, u0));
   return scl_dns_set_velocity(d, a.data(), b.data(), c.data());
 }
 

@@ -18,6 +18,7 @@
 // distinct negative seeds.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <limits>
@@ -172,6 +173,118 @@ int32_t initialize3d(R* vx, R* vy, R* vz, int nx, int ny, int nz, int32_t iseed,
   const int64_t n = 2 * (int64_t)lx1 * ly * lz;
   for (int q = 0; q < 3; ++q)
     for (int64_t i = 0; i < n; ++i) v[q][i] = s * v[q][i];
+  return SCL_OK;
+}
+
+// Helical initial field of initialize_heli.f90:1-109 (spectral_heli_dns.f90's
+// start).  Per mode, in (kk, jj, ii) order with x fastest, two complex
+// Gaussians am, then ap, from ran1 pairs (:22-35); am rescaled on the shells
+// |k - ii| < 1/2 by sqrt((1 - hrel) P/M) with P, M the sums of |ap|^2, |am|^2
+// over |k - ii - oneless/2| < 1/2 (:37-44, the two masks as written there;
+// the sums in FP64); u = sqrt(3/2) (ap h+ + am h-) A(k), A(k) = u0 sqrt(8
+// sqrt(2/pi)/(3 pi akp^5)) k exp(-k^2/akp^2), with the helical basis h+ of
+// :46-88 (i k x h+ = |k| h+, h- = conj h+; its kx-axis form for ky = kz = 0)
+// and h- = conj(h+); then symmetrize and the rescale to sum' |u|^2 =
+// 1.5 u0^2 (:90-101).  At k = 0 the file's basis is 0/0; the mean is zero.
+template <class R>
+int32_t initialize_heli3d(R* ux, R* uy, R* uz, int nx, int ny, int nz, int32_t iseed, double hrel_d, double akp_d,
+                          double u0_d) {
+  const int lx1 = nx / 2 + 1, ly = ny, lz = nz;
+  const int64_t nm = (int64_t)lx1 * ly * lz;
+  const R pi = (R)3.1415926536;  // the reference's literal (constant_sp.f90:20)
+  const R smallest = (R)2 * (R)std::numeric_limits<float>::epsilon();  // constant_sp.f90:7
+  const R oneless = (R)1 - smallest;
+  std::vector<R> am(2 * nm), ap(2 * nm), km(nm);
+  Ran1 ran(iseed);
+  for (int64_t o = 0; o < nm; ++o) {
+    R t1 = ran.uniform<R>(), t2 = (R)2 * pi * ran.uniform<R>();
+    gasdev<R>(t1, t2, am[2 * o], am[2 * o + 1]);
+    t1 = ran.uniform<R>();
+    t2 = (R)2 * pi * ran.uniform<R>();
+    gasdev<R>(t1, t2, ap[2 * o], ap[2 * o + 1]);
+  }
+  for (int kk = 0; kk < lz; ++kk)
+    for (int jj = 0; jj < ly; ++jj)
+      for (int ii = 0; ii < lx1; ++ii) {
+        const int ik2 = ii * ii + kwave(jj, ly) * kwave(jj, ly) + kwave(kk, lz) * kwave(kk, lz);
+        km[((int64_t)kk * ly + jj) * lx1 + ii] = std::sqrt((R)ik2);
+      }
+  const R hrel = (R)hrel_d;
+  for (int sh = 1; sh <= lx1 - 1; ++sh) {
+    double P = 0.0, M = 0.0;
+    for (int64_t o = 0; o < nm; ++o)
+      if (std::fabs(km[o] - (R)sh - (R)0.5 * oneless) < (R)0.5) {
+        P += (double)ap[2 * o] * ap[2 * o] + (double)ap[2 * o + 1] * ap[2 * o + 1];
+        M += (double)am[2 * o] * am[2 * o] + (double)am[2 * o + 1] * am[2 * o + 1];
+      }
+    const R f = M > 0 ? (R)std::sqrt(((double)1 - (double)hrel) * P / M) : (R)0;
+    for (int64_t o = 0; o < nm; ++o)
+      if (std::fabs(km[o] - (R)sh) < (R)0.5) {
+        am[2 * o] *= f;
+        am[2 * o + 1] *= f;
+      }
+  }
+  const R akp = (R)akp_d, u0 = (R)u0_d;
+  const R pref = u0 * std::sqrt((R)8 * std::sqrt((R)2 / pi) / ((R)3 * pi * akp * akp * akp * akp * akp));
+  const R h = std::sqrt((R)2) / (R)2, s32 = std::sqrt((R)3 / (R)2);
+  R* u[3] = {ux, uy, uz};
+  for (int kk = 0; kk < lz; ++kk) {
+    const R kz = (R)kwave(kk, lz);
+    for (int jj = 0; jj < ly; ++jj) {
+      const R ky = (R)kwave(jj, ly);
+      for (int ii = 0; ii < lx1; ++ii) {
+        const R kx = (R)ii;
+        const int64_t o = ((int64_t)kk * ly + jj) * lx1 + ii;
+        const R k = km[o];
+        if (o == 0) {
+          for (int q = 0; q < 3; ++q) u[q][0] = u[q][1] = 0;
+          continue;
+        }
+        const R a = pref * k * std::exp(-k * k / (akp * akp));
+        // h+ = (re, im) per component
+        R hp[3][2];
+        if (jj == 0 && kk == 0) {  // on the kx axis (:50-58)
+          const R k2d = std::max(std::sqrt(kx * kx + ky * ky), smallest);
+          hp[0][0] = h * (-ky / k2d);
+          hp[0][1] = h * (-kx * kz / k / k2d);
+          hp[1][0] = h * (kx / k2d);
+          hp[1][1] = h * (-ky * kz / k / k2d);
+          hp[2][0] = 0;
+          hp[2][1] = h * (ky * ky + kx * kx) / k / k2d;
+        } else {  // (:70-78)
+          const R k2d = std::sqrt(kz * kz + ky * ky);
+          hp[0][0] = 0;
+          hp[0][1] = h * (ky * ky + kz * kz) / k / k2d;
+          hp[1][0] = h * (-kz / k2d);
+          hp[1][1] = h * (-kx * ky / k / k2d);
+          hp[2][0] = h * (ky / k2d);
+          hp[2][1] = h * (-kz * kx / k / k2d);
+        }
+        const R apr = ap[2 * o], api = ap[2 * o + 1], amr = am[2 * o], ami = am[2 * o + 1];
+        for (int q = 0; q < 3; ++q) {
+          const R hr = hp[q][0], hi = hp[q][1];  // h- = (hr, -hi)
+          const R re = (apr * hr - api * hi) + (amr * hr + ami * hi);
+          const R im = (apr * hi + api * hr) + (ami * hr - amr * hi);
+          u[q][2 * o] = s32 * re * a;
+          u[q][2 * o + 1] = s32 * im * a;
+        }
+      }
+    }
+  }
+  for (int q = 0; q < 3; ++q) symmetrize_host<R>(u[q], lx1, ly, lz);
+  double esum = 0.0;
+  for (int64_t o = 0; o < nm; ++o) {
+    double e = 0.0;
+    for (int q = 0; q < 3; ++q) e += (double)u[q][2 * o] * u[q][2 * o] + (double)u[q][2 * o + 1] * u[q][2 * o + 1];
+    esum += (o % lx1) == 0 ? 0.5 * e : e;
+  }
+  if (!(esum > 0)) {
+    set_error("helical initial field has zero energy");
+    return SCL_EINVAL;
+  }
+  const R sc = std::sqrt((R)1.5 * u0 * u0 / (R)esum);
+  for (int q = 0; q < 3; ++q)
+    for (int64_t i = 0; i < 2 * nm; ++i) u[q][i] = sc * u[q][i];
   return SCL_OK;
 }
 

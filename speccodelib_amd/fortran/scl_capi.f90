@@ -110,6 +110,14 @@ module scl_capi
       real(c_double), value :: akp, u0, kcut
     end function
 
+    integer(c_int32_t) function scl_initialize_heli(prec, ux, uy, uz, nx, ny, nz, iseed, hrel, akp, u0) &
+        bind(C, name='scl_initialize_heli')
+      import
+      integer(c_int32_t), value :: prec, nx, ny, nz, iseed
+      type(*), dimension(*) :: ux, uy, uz
+      real(c_double), value :: hrel, akp, u0
+    end function
+
     integer(c_int32_t) function scl_rk4step(plan, vx, vy, vz, nu, dt, forced, fkmax, eps, eta) &
         bind(C, name='scl_rk4step')
       import

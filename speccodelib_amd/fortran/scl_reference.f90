@@ -4,8 +4,8 @@
 ! spectral_dns_scalar.f90 and the drivers of the same family) links these
 ! instead of fftwplan.f90, symmetrize.f90, wavenumber.f90, initialize.f90,
 ! convec_dns.f90, force.f90, force_heli.f90, dealiasing.f90,
-! projection.f90, smag.f90, dynsmag.f90, convecphi_dns.f90 and
-! skewness.f90, and FFTW 2.x: its rfftwnd_f77_one_complex_to_real /
+! projection.f90, smag.f90, dynsmag.f90, convecphi_dns.f90,
+! skewness.f90 and initialize_heli.f90, and FFTW 2.x: its rfftwnd_f77_one_complex_to_real /
 ! _real_to_complex calls resolve to the library's own entry points of those
 ! names.
 !
@@ -221,6 +221,24 @@ subroutine rk4step(vx, vy, vz, lx1, ly, lz, rnu, dt, iforce, fkmax, eps, eta)
   call scl_check(scl_rk4step(c_null_ptr, vx, vy, vz, real(rnu, c_double), real(dt, c_double), iforce, &
                              real(fkmax, c_double), real(eps, c_double), real(eta, c_double)), 'rk4step')
 end subroutine rk4step
+
+! initialize_heli.f90:1-109 (the start of spectral_heli_dns.f90)
+subroutine initialize_heli(ux, uy, uz, kx, ky, kz, k2, lx1, ly, lz, iseed, hrel, akp, u0)
+  use iso_c_binding
+  use mconstant
+  use mfftwplan3d, only: plan_prec
+  use scl_capi
+  implicit none
+  integer, intent(in) :: lx1, ly, lz, iseed
+  real(sp), intent(in) :: akp, u0, hrel
+  complex(sp), dimension(lx1, ly, lz), intent(out) :: ux, uy, uz
+  real(sp), dimension(lx1, ly, lz), intent(in) :: k2
+  real(sp), dimension(lx1), intent(in) :: kx
+  real(sp), dimension(ly), intent(in) :: ky
+  real(sp), dimension(lz), intent(in) :: kz
+  call scl_check(scl_initialize_heli(plan_prec(), ux, uy, uz, 2 * (lx1 - 1), ly, lz, iseed, real(hrel, c_double), &
+                                     real(akp, c_double), real(u0, c_double)), 'initialize_heli')
+end subroutine initialize_heli
 
 ! dynsmag.f90:1-236.  The logical update and the real(sp) mm, lm, cs2 of the
 ! reference become the C ABI's int32 and doubles here; with update false

@@ -13,6 +13,7 @@
 !   mode 6: the library's derivs (scl_derivs, bound to an FP64 solver)
 !           stepped by a classical RK4 driver written here against the
 !           derivs(x, y, dydx, ny) interface of rk5.f90:3-10
+!   mode 7: initialize_heli (seed -9, hrel 0.6, akp 4, u0 1) only
 ! Writes vx, vy, vz (complex(sp), (lx1,ly,lz) each; mode 4 also phi; mode 6
 ! complex(dp)) to outfile as a stream.
 program dns_deform_gpu
@@ -60,6 +61,16 @@ program dns_deform_gpu
     call derivs_run()
     call destroyplan3d
     write(*, '(a,i0,a,i0,a,i0)') 'dns_deform_gpu done: nx=', nx, ' steps=', nsteps, ' mode=', mode
+    stop
+  end if
+  if (mode == 7) then
+    ! the helical start of spectral_heli_dns.f90:92 (seed -9, hrel 0.6)
+    call initialize_heli(vx, vy, vz, kx, ky, kz, k2, lx1, ly, lz, -9, 0.6_sp, akp, u0)
+    open(11, file=trim(outfile), access='stream', form='unformatted', status='replace')
+    write(11) vx, vy, vz
+    close(11)
+    call destroyplan3d
+    write(*, '(a,i0,a,i0)') 'dns_deform_gpu done: nx=', nx, ' mode=', mode
     stop
   end if
 

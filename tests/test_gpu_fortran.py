@@ -140,3 +140,18 @@ def test_fortran_padded_routines(tmp_path):
     e2 = max(np.abs(x - y).max() for x, y in zip(tau, tw)) / max(np.abs(y).max() for y in tw)
     print(f"padded convec {e1:.2e}, smag_dealias {e2:.2e}", flush=True)
     assert e1 < 2e-6 and e2 < 2e-6
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built (amdflang)")
+def test_fortran_initialize_heli(tmp_path):
+    """initialize_heli by its reference name (mode 7), FP32, against the
+    FP64 oracle."""
+    n = 16
+    out = tmp_path / "heli.bin"
+    print("case fortran initialize_heli", flush=True)
+    _run(7, n, 0, out)
+    v = np.fromfile(out, np.complex64).reshape(3, n, n, n // 2 + 1)
+    want = orc.initialize_heli(n, n, n, -9, 0.6, 4.0, 1.0)
+    err = max(np.abs(a - b).max() for a, b in zip(v, want)) / max(np.abs(b).max() for b in want)
+    print(f"initialize_heli fp32 rel err {err:.2e}", flush=True)
+    assert err < 2e-5

@@ -4,8 +4,21 @@ ran1 = Park-Miller minimal standard LCG (x <- 16807 x mod 2^31-1) behind a
 Bays-Durham shuffle.  The LCG is pinned by Park & Miller's published check
 value (x_10000 = 1043618065 from x_0 = 1); the shuffle only delays values, so
 every output must be one of the LCG values drawn so far.
+
+initialize_heli (initialize_heli.f90:1-109) is pinned by what its algebra
+fixes: a solenoidal field; with hrel = 1 every mode of the rescaled shells
+satisfies i k x u = |k| u (maximal positive helicity); sum' |u|^2 = 1.5 u0^2;
+and the ratio of two modes of one shell, whose basis factors are closed-form,
+against ran1 pairs drawn here in the file's order (am before ap).  The
+library's host routine matches it (DP 1e-12, SP 2e-5).
 """
+import inspect
+
 import numpy as np
+import pytest
+
+import speccodelib_amd as scl
+from oracle import spectral3d as orc
 
 
 def lcg(x0, n):
@@ -63,3 +76,76 @@ def test_sp_and_dp_initial_fields_agree(oracle):
     for x, y in zip(a, b):
         assert y.dtype == np.complex64
         assert np.abs(x - y).max() < 1e-5 * np.abs(x).max()
+
+
+# --------------------------------------------------------------------------
+# initialize_heli (initialize_heli.f90:1-109)
+def _curl_i(u, n):
+    kx, ky, kz, _ = orc.wavenumber(n, n, n)
+    K = [kx[None, None, :], ky[None, :, None], kz[:, None, None]]
+    return [1j * (K[1] * u[2] - K[2] * u[1]), 1j * (K[2] * u[0] - K[0] * u[2]), 1j * (K[0] * u[1] - K[1] * u[0])], K
+
+
+def _check_heli(fn):
+    n = 16
+    u = fn(n, n, n, -5, 1.0, 4.0, 1.0)
+    w, K = _curl_i(u, n)
+    kk = np.sqrt(np.round(orc.wavenumber(n, n, n)[3]))
+    umax = max(np.abs(a).max() for a in u)
+    # solenoidal
+    div = K[0] * u[0] + K[1] * u[1] + K[2] * u[2]
+    assert np.abs(div).max() < 1e-13 * umax * n
+    # hrel = 1: every mode of the rescaled shells (|k| < lx + 1/2) is
+    # positively helical, i k x u = |k| u
+    m = kk < n // 2 + 0.5
+    err = max(np.abs((wq - kk * uq)[m]).max() for wq, uq in zip(w, u)) / (umax * n)
+    assert err < 1e-13, err
+    # sum' |u|^2 = 1.5 u0^2
+    assert abs(orc.energy(*u) - 1.5) < 1e-13
+    # draw order: modes (3,4,0) and (4,0,3) (|k| = 5) hold ap h+ A(5) with
+    # h+_x = i sqrt(2)/2 sqrt(ky^2 + kz^2)/|k|, ap from ran1 pairs 3 and 4 of
+    # the mode (am first); so u_x(3,4,0)/u_x(4,0,3) = (4/3) ap(3,4,0)/ap(4,0,3)
+    lx1 = n // 2 + 1
+    d = orc.ran1_stream(-5, 4 * n * n * lx1).astype(np.float64)
+
+    def ap(ix, iy, iz):
+        o = (iz * n + iy) * lx1 + ix
+        t1, t2 = [min(x / 2147483647.0, 1.0 - 1.2e-7) for x in d[4 * o + 2: 4 * o + 4]]
+        return np.sqrt(-2 * np.log(t1)) * np.exp(2j * orc.PI_REF * t2) / np.sqrt(2)
+
+    r = u[0][0, 4, 3] / u[0][3, 0, 4]
+    assert abs(r - (4 / 3) * ap(3, 4, 0) / ap(4, 0, 3)) < 1e-12 * abs(r)
+
+
+def test_initialize_heli():
+    _check_heli(orc.initialize_heli)
+    # hrel = 0 is not maximally helical
+    n = 16
+    u = orc.initialize_heli(n, n, n, -5, 0.0, 4.0, 1.0)
+    w, _ = _curl_i(u, n)
+    kk = np.sqrt(np.round(orc.wavenumber(n, n, n)[3]))
+    assert max(np.abs(wq - kk * uq).max() for wq, uq in zip(w, u)) > 1e-2 * max(np.abs(a).max() for a in u)
+
+
+@pytest.mark.parametrize("old,new", [
+    ("am = g[:, 0].reshape(nz, ny, lx1).copy()\n    ap = g[:, 1]", "am = g[:, 1].reshape(nz, ny, lx1).copy()\n    ap = g[:, 0]"),
+    ("h * (-KZ / k2d - 1j * KX * KY / kk / k2d)", "h * (KZ / k2d - 1j * KX * KY / kk / k2d)"),
+    ("h * 1j * (KY**2 + KZ**2) / kk / k2d)", "h * (KY**2 + KZ**2) / kk / k2d)"),
+])
+def test_initialize_heli_check_catches_planted_mistakes(old, new):
+    src = inspect.getsource(orc.initialize_heli)
+    assert old in src, old
+    ns = dict(orc.__dict__)
+    exec(compile(src.replace(old, new), "<mutant>", "exec"), ns)
+    with pytest.raises(AssertionError):
+        _check_heli(ns["initialize_heli"])
+
+
+@pytest.mark.parametrize("prec,tol", [(scl.DP, 1e-12), (scl.SP, 2e-5)])
+def test_library_initialize_heli_matches_oracle(prec, tol):
+    n = 16
+    want = orc.initialize_heli(n, n, n, -9, 0.6, 4.0, 1.0)
+    got = scl.initialize_heli(n, n, n, -9, 0.6, 4.0, 1.0, prec)
+    err = max(np.abs(np.asarray(a, np.complex128) - b).max() for a, b in zip(got, want)) / max(
+        np.abs(b).max() for b in want)
+    assert err < tol, err
