@@ -111,3 +111,36 @@ def test_fp32_run_and_graph_bitwise():
     print(f"fp32 20 steps rel err {err:.2e}", flush=True)
     assert err < 2e-5
     p.close()
+
+
+def test_heli_dns_loop_fp64():
+    """spectral_heli_dns.f90's loop: the initialize_heli start, force_heli in
+    the RHS, IF-AB2 with CFL steps; FP64 against the oracle (1e-11)."""
+    n, nsteps, eps, eta = 32, 12, 0.1, 0.05
+    print("case ab2 helical dns fp64", flush=True)
+    v0 = orc.initialize_heli(n, n, n, -3, 0.8, 4.0, 1.0)
+    d = orc.DNS(orc.DNSParams(nx=n, nu=CFL["nu"], dt=0.0, forced=False))
+
+    def rhs(vx, vy, vz):
+        om = orc.vorticity(vx, vy, vz, d.kx, d.ky, d.kz)
+        nl = orc.convec_dns(vx, vy, vz, *om, n, n, n)
+        f = orc.force_heli(vx, vy, vz, *om, d.k2, 2.0, eps, eta)
+        w = [a + b for a, b in zip(nl, f)]
+        for a in w:
+            orc.symmetrize3d(a)
+        orc.projection(*w, d.kx, d.ky, d.kz)
+        return list(w)
+
+    kw = {k: v for k, v in CFL.items() if k not in ("forced", "fkmax", "eps")}
+    o = ab2.AB2(ab2.AB2Params(nx=n, kcut=n // 2 - 1.0, forced=False, **kw), v0, rhs=rhs)
+    p = scl.Plan3D(n, n, n, scl.DP)
+    s = scl.DNS(p, forced=2, fkmax=2.0, eps=eps, eta=eta, stepper=1, kcut=n // 2 - 1.0, **kw)
+    s.initialize_heli(-3, 0.8, 4.0, 1.0)
+    for _ in range(nsteps):
+        s.step(1)
+        o.step()
+    err = relerr(s.get_velocity(), o.v)
+    print(f"helical ab2 rel err {err:.2e}, t {s.time} vs {o.time}", flush=True)
+    assert err < 1e-11 and abs(s.time - o.time) < 1e-14
+    s.close()
+    p.close()
