@@ -206,6 +206,14 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
 // 2 waves per SIMD: LDS to spare in both): Uz waits in park (E BS entries,
 // each thread reading back only its own) between its store and its two
 // later uses.
+// An int the compiler cannot see through: the LDS addresses a transform
+// derives from it are recomputed for that transform instead of being kept
+// live (and spilled) from the previous one.
+__device__ inline int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <class R, int N, int E, int CW, bool PARK, int ZS>
 __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds, cpx<R>* park,
                                   const cpx<R>* twl, int rt, int c, R kx, R ky, int nz, ZOff<ZS> zo,
@@ -220,7 +228,7 @@ __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)
   auto kz = [&](int m) { return (R)kwave(rt + T * m, nz); };
   const int slot = rt * CW + c;
   cpx<R> d[E];
-  L::run(uz, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(uz, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     (w + 2 * ws)[off(m)] = uz[m];
@@ -229,18 +237,18 @@ __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)
   auto Uz = [&](int m) { return PARK ? park[m * BS + slot] : uz[m]; };
 #pragma unroll
   for (int m = 0; m < E; ++m) d[m] = kz(m) * uy[m];
-  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(d, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
 #pragma unroll
   for (int m = 0; m < E; ++m) (w + 3 * ws)[off(m)] = mul_i(ky * Uz(m) - d[m]);
-  L::run(uy, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(uy, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
 #pragma unroll
   for (int m = 0; m < E; ++m) (w + ws)[off(m)] = uy[m];
 #pragma unroll
   for (int m = 0; m < E; ++m) d[m] = kz(m) * ux[m];
-  L::run(d, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(d, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
 #pragma unroll
   for (int m = 0; m < E; ++m) (w + 4 * ws)[off(m)] = mul_i(d[m] - kx * Uz(m));
-  L::run(ux, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(ux, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
 #pragma unroll
   for (int m = 0; m < E; ++m) w[off(m)] = ux[m];
 }
@@ -312,27 +320,49 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_yinv_vort(cpx<R>* __rest
   store(b, 1);
 }
 
+// The values one stage update of one component reads: v and the
+// accumulator du (IF-RK4, MODE 1..4) or v and the AB2 history (MODE 5..7:
+// v0 from DU, the last N from S).  Loaded apart from the update so that a
+// caller can issue them ahead of the stores of the slots before.
+template <class R>
+struct StageIn {
+  cpx<R> v, h;
+};
+
+// MODE 4 reads du only inside the cut.  NT: nontemporal loads (whole-line
+// accesses only, ld_once).
+template <int MODE, class R, class I, bool NT = false>
+__device__ inline StageIn<R> stage_load(I o, const cpx<R>* V, const cpx<R>* DU, const cpx<R>* S, bool cut) {
+  StageIn<R> in{mk<R>(0, 0), mk<R>(0, 0)};
+  if constexpr (MODE != 4) in.v = ld_once<R, NT>(V + o);
+  if constexpr (MODE == 2 || MODE == 3 || MODE == 6) in.h = ld_once<R, NT>(DU + o);
+  if constexpr (MODE == 4) {
+    if (!cut) in.h = ld_once<R, NT>(DU + o);
+  }
+  if constexpr (MODE == 7) in.h = ld_once<R, NT>(S + o);
+  return in;
+}
+
 // One IF-RK4 stage of one component in the division-free form (see
 // k_zfwd_rhs): a = dt N, em1 = expm1(x), x = -rate dt/2, so E y = y + em1 y
 // and E^2 y = y + expm1(2x) y, each exact to the rounding of the result.
 // Returns the new stage state (MODE 1..3, stored to S only where store_s)
-// or the new v (MODE 4, always stored).
+// or the new v (MODE 4, always stored); in holds what stage_load read.
 // MODE 4 with nf set: a non-finite new v raises the sticky flag *nf (read
 // first, so that a field full of NaN does not make every lane an atomic).
-// NT: nontemporal loads of v and du (whole-line accesses only, ld_once).
-template <int MODE, class R, class I, bool NT = false>
-__device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
-                                    bool store_s, unsigned* nf = nullptr) {
+template <int MODE, class R, class I>
+__device__ inline cpx<R> stage_apply(const StageIn<R>& in, cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1,
+                                     R x, bool cut, bool store_s, unsigned* nf = nullptr) {
   cpx<R> out;
   if constexpr (MODE == 1) {
-    const cpx<R> v = ld_once<R, NT>(V + o);
+    const cpx<R> v = in.v;
     const cpx<R> u = v + ((R)1 / (R)6) * a, q = v + (R)0.5 * a;
     DU[o] = u + expm1_r((R)2 * x) * u;
     out = q + em1 * q;
   } else if constexpr (MODE == 2 || MODE == 3) {
-    const cpx<R> v = ld_once<R, NT>(V + o);
+    const cpx<R> v = in.v;
     const cpx<R> q = ((R)1 / (R)3) * a;
-    DU[o] = ld_once<R, NT>(DU + o) + (q + em1 * q);
+    DU[o] = in.h + (q + em1 * q);
     const cpx<R> ev = v + em1 * v;
     if constexpr (MODE == 2) {
       out = ev + (R)0.5 * a;
@@ -341,13 +371,20 @@ __device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>
       out = u + em1 * u;
     }
   } else {
-    out = cut ? mk<R>(0, 0) : ld_once<R, NT>(DU + o) + ((R)1 / (R)6) * a;
+    out = cut ? mk<R>(0, 0) : in.h + ((R)1 / (R)6) * a;
     V[o] = out;
     if (nf && !(isfinite(out.x) && isfinite(out.y)) && *(volatile unsigned*)nf == 0u) atomicOr(nf, 1u);
     return out;
   }
   if (store_s) S[o] = out;
   return out;
+}
+
+template <int MODE, class R, class I, bool NT = false>
+__device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
+                                    bool store_s, unsigned* nf = nullptr) {
+  return stage_apply<MODE, R>(stage_load<MODE, R, I, NT>(o, V, DU, S, cut), a, o, V, DU, S, em1, x, cut, store_s,
+                              nf);
 }
 
 template <int MODE, class R, class I>
@@ -382,20 +419,20 @@ enum { AB_TIME = 0, AB_DT = 1, AB_DTO = 2, AB_UMAX = 3, AB_TWRITE = 4, AB_ISTEP 
 //   MODE 6 (step 1):  v2 = [cS E(dto) v0 + cek v1 + cdel a] E(dt)  (v0 from DU)
 //   MODE 7 (>= 2):    v = [v + cS a + cek E(dto) O] E(dt);  S = a (O = S)
 // The new v is 0 where cut (k2 >= kcut^2, and the mean).  nf: as stage_keep.
-template <int MODE, class R, class I, bool NT = false>
-__device__ inline cpx<R> ab_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS, R cek, R cdel, R e1,
-                                 R e2, bool cut, unsigned* nf) {
-  const cpx<R> v = ld_once<R, NT>(V + o);
+template <int MODE, class R, class I>
+__device__ inline cpx<R> ab_apply(const StageIn<R>& in, cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS,
+                                  R cek, R cdel, R e1, R e2, bool cut, unsigned* nf) {
+  const cpx<R> v = in.v;
   cpx<R> y;
   if constexpr (MODE == 5) {
     DU[o] = v;
     S[o] = a;
     y = v + dt * a;
   } else if constexpr (MODE == 6) {
-    const cpx<R> v0 = ld_once<R, NT>(DU + o);
+    const cpx<R> v0 = in.h;
     y = cS * (v0 + e2 * v0) + cek * v + cdel * a;
   } else {
-    const cpx<R> ob = ld_once<R, NT>(S + o);
+    const cpx<R> ob = in.h;
     y = v + cS * a + cek * (ob + e2 * ob);
     S[o] = a;
   }
@@ -403,6 +440,13 @@ __device__ inline cpx<R> ab_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S
   V[o] = out;
   if (nf && !(isfinite(out.x) && isfinite(out.y)) && *(volatile unsigned*)nf == 0u) atomicOr(nf, 1u);
   return out;
+}
+
+template <int MODE, class R, class I, bool NT = false>
+__device__ inline cpx<R> ab_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS, R cek, R cdel, R e1,
+                                 R e2, bool cut, unsigned* nf) {
+  return ab_apply<MODE, R>(stage_load<MODE, R, I, NT>(o, V, DU, S, cut), a, o, V, DU, S, dt, cS, cek, cdel, e1, e2,
+                           cut, nf);
 }
 
 // ZFWD: z-forward of the three transformed products + forcing + the
@@ -458,16 +502,34 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     if constexpr (!PARK) nz_[m] = ld_once<R, (CW >= 16)>(W2 + o);
   }
   using L = Line<R, N, E, -1>;
-  L::run(nx_, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(nx_, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
   if constexpr (PARK) {
+    // offsets recomputed, not kept live through the transform above
+    ZOff<ZS> zn = zo;
+    asm volatile("" : "+v"(zn.ob));
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       park[m * CC::BS + threadIdx.x] = nx_[m];
-      nz_[m] = ld_once<R, (CW >= 16)>(W2 + zo.at(rt + T * m));
+      nz_[m] = ld_once<R, (CW >= 16)>(W2 + zn.at(rt + T * m));
     }
   }
-  L::run(ny_, lds, rt, twl, 1, ColIdx<CW>{c});
-  L::run(nz_, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(ny_, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
+  // The stage update's loads (v, du or the AB2 history) of all three
+  // components of slot m are issued together, before any of the slot's
+  // stores: V, DU and S may alias as far as the compiler knows, so it would
+  // otherwise wait out one load round trip per component.
+  constexpr bool NT = CW >= 16;
+  auto stage_in = [&](int m, const ZOff<ZS>& zz, StageIn<R> (&dst)[3]) {
+    const int z = L::out_pos(m, rt);
+    const int iy = kwave(t.oth, g.ny), iz = kwave(z, g.nz);
+    const int ik2 = t.kx * t.kx + iy * iy + iz * iz;
+    const bool cut = (ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2) >= p.kc2;  // what MODE 4 reads by
+    const uint32_t o = zz.at(z);
+    dst[0] = stage_load<MODE, R, uint32_t, NT>(o, V, DU, S, cut);
+    dst[1] = stage_load<MODE, R, uint32_t, NT>(o, V + fstride, DU + fstride, S + fstride, cut);
+    dst[2] = stage_load<MODE, R, uint32_t, NT>(o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, cut);
+  };
+  L::run(nz_, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
   // the epilogue recomputes its offsets (not kept live from the loads)
   ZOff<ZS> ze = zo;
   asm volatile("" : "+v"(ze.ob));
@@ -489,8 +551,16 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     ab_ek = (R)p.ab[AB_EK];
     ab_del = (R)p.ab[AB_DELTA];
   }
+  // With the park, the new x stage state goes back to its park slot (just
+  // read) until the z-inverse, which keeps 2E registers free here.
+  auto keep_x = [&](int m, cpx<R> v) {
+    if constexpr (PARK && FUSED) park[m * CC::BS + threadIdx.x] = v;
+    else nx_[m] = v;
+  };
 #pragma unroll
   for (int m = 0; m < E; ++m) {
+    StageIn<R> in[3];
+    if constexpr (MODE > 0) stage_in(m, ze, in);
     const int z = L::out_pos(m, rt);
     const int ikz = kwave(z, g.nz);
     const R kz = (R)ikz;
@@ -538,12 +608,11 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R e1 = expm1_r(xk * ab_dt);
       const R e2 = MODE == 5 ? (R)0 : expm1_r(xk * ab_dto);
       const bool cut = k2 >= p.kc2 || ik2 == 0;
-      constexpr bool NT = CW >= 16;
-      nx_[m] = ab_keep<MODE, R, uint32_t, NT>(a, o, V, DU, S, ab_dt, ab_s, ab_ek, ab_del, e1, e2, cut, p.nonfinite);
-      ny_[m] = ab_keep<MODE, R, uint32_t, NT>(b, o, V + fstride, DU + fstride, S + fstride, ab_dt, ab_s, ab_ek, ab_del,
-                                              e1, e2, cut, p.nonfinite);
-      nz_[m] = ab_keep<MODE, R, uint32_t, NT>(d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, ab_dt, ab_s,
-                                              ab_ek, ab_del, e1, e2, cut, p.nonfinite);
+      keep_x(m, ab_apply<MODE, R>(in[0], a, o, V, DU, S, ab_dt, ab_s, ab_ek, ab_del, e1, e2, cut, p.nonfinite));
+      ny_[m] = ab_apply<MODE, R>(in[1], b, o, V + fstride, DU + fstride, S + fstride, ab_dt, ab_s, ab_ek, ab_del, e1,
+                                 e2, cut, p.nonfinite);
+      nz_[m] = ab_apply<MODE, R>(in[2], d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, ab_dt, ab_s, ab_ek,
+                                 ab_del, e1, e2, cut, p.nonfinite);
     } else {
       // E = exp(x), x = -nu k2 dt/2.  E is within 1e-3 of 1 for most modes,
       // where FP32 spacing (6e-8) makes a rounded E or E*E a biased factor
@@ -553,15 +622,18 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
       const R em1 = expm1_r(x);
       const bool cut = k2 >= p.kc2;
       const bool store_s = !FUSED || (p.forced && k2 < p.fk2);
-      constexpr bool NT = CW >= 16;
-      nx_[m] = stage_keep<MODE, R, uint32_t, NT>(p.dt * a, o, V, DU, S, em1, x, cut, store_s, p.nonfinite);
-      ny_[m] = stage_keep<MODE, R, uint32_t, NT>(p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut,
-                                                 store_s, p.nonfinite);
-      nz_[m] = stage_keep<MODE, R, uint32_t, NT>(p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1,
-                                                 x, cut, store_s, p.nonfinite);
+      keep_x(m, stage_apply<MODE, R>(in[0], p.dt * a, o, V, DU, S, em1, x, cut, store_s, p.nonfinite));
+      ny_[m] = stage_apply<MODE, R>(in[1], p.dt * b, o, V + fstride, DU + fstride, S + fstride, em1, x, cut, store_s,
+                                    p.nonfinite);
+      nz_[m] = stage_apply<MODE, R>(in[2], p.dt * d, o, V + 2 * fstride, DU + 2 * fstride, S + 2 * fstride, em1, x,
+                                    cut, store_s, p.nonfinite);
     }
   }
   if constexpr (FUSED && MODE > 0) {
+    if constexpr (PARK) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) nx_[m] = park[m * CC::BS + threadIdx.x];
+    }
     zinv_store<R, N, E, CW, PARK, ZS>(nx_, ny_, nz_, lds, park, twl, rt, c, kx, ky, g.nz, zo, W, fstride);
   }
 }
