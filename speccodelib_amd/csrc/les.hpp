@@ -252,20 +252,27 @@ struct ExtParams {
 // weights this step's k_ab_begin left in r.ab) as spectral_les.f90:241-304
 // and spectral_dns_scalar.f90:257-338 write it, with the component's
 // viscosity nu (nu for u, nu/Sc for the scalar); AB2 also zeroes the mean.
+// in holds what stage_load read for the component (ext_update loads it).
 template <int MODE, class R, class I>
-__device__ inline void ext_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R nu, R k2, bool cut, bool mean,
-                                  const RhsParams<R>& r) {
+__device__ inline void ext_apply(const StageIn<R>& in, cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R nu, R k2,
+                                 bool cut, bool mean, const RhsParams<R>& r) {
   if constexpr (MODE >= 5) {
     const R dt = (R)r.ab[AB_DT];
     const R xk = -nu * k2;
     const R e1 = expm1_r(xk * dt);
     const R e2 = MODE == 5 ? (R)0 : expm1_r(xk * (R)r.ab[AB_DTO]);
-    (void)ab_keep<MODE, R, I, false>(a, o, V, DU, S, dt, (R)r.ab[AB_S], (R)r.ab[AB_EK], (R)r.ab[AB_DELTA], e1, e2,
-                                      cut || mean, r.nonfinite);
+    (void)ab_apply<MODE, R>(in, a, o, V, DU, S, dt, (R)r.ab[AB_S], (R)r.ab[AB_EK], (R)r.ab[AB_DELTA], e1, e2,
+                            cut || mean, r.nonfinite);
   } else {
     const R x = -nu * k2 * r.dt / (R)2;
-    stage_update<MODE, R>(r.dt * a, o, V, DU, S, expm1_r(x), x, cut, r.nonfinite);
+    (void)stage_apply<MODE, R>(in, r.dt * a, o, V, DU, S, expm1_r(x), x, cut, true, r.nonfinite);
   }
+}
+
+template <int MODE, class R, class I>
+__device__ inline void ext_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R nu, R k2, bool cut, bool mean,
+                                  const RhsParams<R>& r) {
+  ext_apply<MODE, R>(stage_load<MODE, R, I>(o, V, DU, S, cut), a, o, V, DU, S, nu, k2, cut, mean, r);
 }
 
 // Combine (spectral): N = u x w + f - i k_j tau_ij (spectral_les.f90:232-234),
@@ -653,6 +660,16 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zfwd(cpx<R>* W, int6
     const int ik2 = ikx * ikx + iky * iky + ikz * ikz;
     const R k2 = ik2 == 0 ? tiny : (R)ik2;
     const int64_t o = zo.at(z);
+    // the stage update's loads of all components of this slot are issued
+    // together, before any of its stores (V, DU, S may alias as far as the
+    // compiler knows, which would otherwise cost a round trip per component)
+    StageIn<R> in[4];
+    if constexpr (MODE > 0) {
+      const bool cut = k2 >= p.r.kc2;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) in[q] = stage_load<MODE, R, int64_t>(o + q * fs, V, DU, S, cut);
+      if (L.scalar) in[3] = stage_load<MODE, R, int64_t>(o + 3 * fs, V, DU, S, cut);
+    }
     cpx<R> ax = a[m], by = b[m], cz = dz[m];
     if (p.r.forced && k2 < p.r.fk2) {
       const cpx<R> sx = st[o], sy = st[o + fs], sz = st[o + 2 * fs];
@@ -696,10 +713,10 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_ext_zfwd(cpx<R>* W, int6
       if (L.scalar) W[o + 3 * fs] = r;
     } else {
       const bool cut = k2 >= p.r.kc2, mean = ik2 == 0;
-      ext_update<MODE, R>(ax, o, V, DU, S, p.r.nu, k2, cut, mean, p.r);
-      ext_update<MODE, R>(by, o + fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
-      ext_update<MODE, R>(cz, o + 2 * fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
-      if (L.scalar) ext_update<MODE, R>(r, o + 3 * fs, V, DU, S, p.nu_phi, k2, cut, mean, p.r);
+      ext_apply<MODE, R>(in[0], ax, o, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+      ext_apply<MODE, R>(in[1], by, o + fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+      ext_apply<MODE, R>(in[2], cz, o + 2 * fs, V, DU, S, p.r.nu, k2, cut, mean, p.r);
+      if (L.scalar) ext_apply<MODE, R>(in[3], r, o + 3 * fs, V, DU, S, p.nu_phi, k2, cut, mean, p.r);
     }
   }
 }
