@@ -206,14 +206,6 @@ __global__ __launch_bounds__((ColCfg<R, N>::BS)) void k_colfft(cpx<R>* __restric
 // 2 waves per SIMD: LDS to spare in both): Uz waits in park (E BS entries,
 // each thread reading back only its own) between its store and its two
 // later uses.
-// An int the compiler cannot see through: the LDS addresses a transform
-// derives from it are recomputed for that transform instead of being kept
-// live (and spilled) from the previous one.
-__device__ inline int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
 template <class R, int N, int E, int CW, bool PARK, int ZS>
 __device__ inline void zinv_store(cpx<R> (&ux)[E], cpx<R> (&uy)[E], cpx<R> (&uz)[E], cpx<R>* lds, cpx<R>* park,
                                   const cpx<R>* twl, int rt, int c, R kx, R ky, int nz, ZOff<ZS> zo,

@@ -83,6 +83,14 @@ __device__ inline cpx<R> ld_once(const cpx<R>* p) {
   }
 }
 
+// An int the compiler cannot see through: the LDS addresses a line
+// transform derives from it are recomputed for that transform instead of
+// being kept live (and spilled) from the previous one.
+__device__ inline int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Geometry of a 3D plan, shared by all kernels.  Row (y, z) holds the mx
 // main entries of that (y, z) and, at nmain + row, its kx = nx/2 entry.
 // Rows are ordered [z >> zs][y][z & (2^zs - 1)]: with zs = 3, the 8

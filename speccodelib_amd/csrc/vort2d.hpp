@@ -104,13 +104,13 @@ __device__ inline void yinv2d_store(cpx<R> (&o)[E], cpx<R>* lds, const cpx<R>* t
     const R ky = (R)kwave(rt + T * m, N);
     a[m] = mk<R>(-(ky * o[m].y) / ik2r(m), (ky * o[m].x) / ik2r(m));
   }
-  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(a, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
   store(a, 0);
 #pragma unroll
   for (int m = 0; m < E; ++m) a[m] = mk<R>((kx * o[m].y) / ik2r(m), -(kx * o[m].x) / ik2r(m));
-  L::run(a, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(a, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
   store(a, 1);
-  L::run(o, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(o, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
   store(o, 2);
 }
 
@@ -242,8 +242,8 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
     f0[m] = t.ok ? W[o] : mk<R>(0, 0);
     f1[m] = t.ok ? W[o + fstride] : mk<R>(0, 0);
   }
-  L::run(f0, lds, rt, twl, 1, ColIdx<CW>{c});
-  L::run(f1, lds, rt, twl, 1, ColIdx<CW>{c});
+  L::run(f0, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
+  L::run(f1, lds, opaque(rt), twl, 1, ColIdx<CW>{c});
 
   const R tiny = std::numeric_limits<R>::min();
   const int ikx = t.kx;
@@ -272,8 +272,26 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
     }
   };
   cpx<R> ns[FUSED ? E : 1];  // the new stage vorticity (FUSED)
+  // The stage state w and accumulator of slot m + 1 are loaded before the
+  // stores of slot m (which the compiler would otherwise not move them
+  // above), so each slot's loads are in flight through the slot before.
+  auto stage_in = [&](int m, cpx<R>& wv, cpx<R>& acc) {
+    wv = mk<R>(0, 0);
+    acc = wv;
+    if (mode == 0 || !t.ok) return;
+    const int y = L::out_pos(m, rt);
+    const int iky = kwave(y, N);
+    const int ik2 = ikx * ikx + iky * iky;
+    const int64_t o = t.base + (int64_t)y * t.rs;
+    if (mode <= 3) wv = WS[o];
+    if (mode == 2 || mode == 3 || (mode == 4 && !((ik2 == 0 ? tiny : (R)ik2) >= p.kc2))) acc = ACC[o];
+  };
+  cpx<R> wv_c, acc_c;
+  stage_in(0, wv_c, acc_c);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
+    cpx<R> wv_n, acc_n;
+    if (m + 1 < E) stage_in(m + 1, wv_n, acc_n);
     const int y = L::out_pos(m, rt);
     const int iky = kwave(y, N);
     const R ky = (R)iky;
@@ -296,7 +314,7 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
     if (mode != 0 && t.ok) {
       const R x = -p.rate(k2) * p.dt / (R)2;
       const R em1 = expm1_r(x);
-      const cpx<R> wv = WS[o];
+      const cpx<R> wv = wv_c;
       const cpx<R> a = p.dt * r;
       if (mode == 1) {
         const cpx<R> u = wv + ((R)1 / (R)6) * a, q = wv + (R)0.5 * a;
@@ -304,7 +322,7 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
         o2 = q + em1 * q;
       } else if (mode == 2 || mode == 3) {
         const cpx<R> q = ((R)1 / (R)3) * a;
-        o1 = ACC[o] + (q + em1 * q);
+        o1 = acc_c + (q + em1 * q);
         const cpx<R> ew = wv + em1 * wv;
         if (mode == 2) {
           o2 = ew + (R)0.5 * a;
@@ -313,7 +331,7 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
           o2 = u + em1 * u;
         }
       } else {
-        o1 = k2 >= p.kc2 ? mk<R>(0, 0) : ACC[o] + ((R)1 / (R)6) * a;
+        o1 = k2 >= p.kc2 ? mk<R>(0, 0) : acc_c + ((R)1 / (R)6) * a;
       }
     }
     if (t.ok) put(y, o1, o2);
@@ -322,6 +340,8 @@ __global__ __launch_bounds__((Col2D<R, N, E, CW>::BS)) void k_yfwd2d(cpx<R>* W, 
       lds[y] = o1;
       lds[N + y] = o2;
     }
+    wv_c = wv_n;
+    acc_c = acc_n;
   }
   if (kx0_tile) {
     __syncthreads();
