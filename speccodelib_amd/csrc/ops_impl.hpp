@@ -14,7 +14,7 @@ using IC = std::integral_constant<int, V>;
 // Column/line lengths supported per precision.
 template <class R>
 constexpr int max_line() {
-  return sizeof(R) == 4 ? 1024 : 256;
+  return sizeof(R) == 4 ? 1024 : 512;
 }
 
 template <class R, class F>

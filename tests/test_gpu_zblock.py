@@ -77,4 +77,5 @@ def test_adjoint_fp64():
 
 
 def test_long_z_lines_512():
-    zl.test_long_z_lines_rhs_and_step((32, 32, 512))
+    zl.test_long_z_lines_rhs_and_step((32, 32, 512), scl.SP)
+    zl.test_long_z_lines_rhs_and_step((32, 32, 512), scl.DP)
