@@ -10,7 +10,7 @@
  * are passed by value.  The reference's Fortran module set (fortran/) binds
  * these through iso_c_binding under the reference's own procedure names.
  *
- * Grids: nx, ny, nz powers of two in [16, 1024] (FP64 plans: [16, 512]).
+ * Grids: nx, ny, nz powers of two in [16, 1024] (FP32 and FP64 plans).
  * The wavenumber arrays a reference routine takes (kx, ky, kz, k2) must be
  * the wavenumber() ones (wavenumber.f90:89-129); the library checks the 1D
  * arrays and builds k2 itself.

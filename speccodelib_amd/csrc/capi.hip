@@ -222,10 +222,9 @@ extern "C" int32_t scl_version(void) { return 100; }
 extern "C" int32_t scl_plan3d_create(int32_t nx, int32_t ny, int32_t nz, int32_t prec, scl_plan3d** out) {
   if (!out) return set_error("null plan pointer"), SCL_EINVAL;
   *out = nullptr;
-  const int hi = prec == SCL_SP ? 1024 : 512;
   if (prec != SCL_SP && prec != SCL_DP) return set_error("bad precision %d", prec), SCL_EINVAL;
-  if (!pow2_in(nx, 16, hi) || !pow2_in(ny, 16, hi) || !pow2_in(nz, 16, hi))
-    return set_error("grid (%d,%d,%d): each size must be a power of two in [16,%d]", nx, ny, nz, hi), SCL_EINVAL;
+  if (!pow2_in(nx, 16, 1024) || !pow2_in(ny, 16, 1024) || !pow2_in(nz, 16, 1024))
+    return set_error("grid (%d,%d,%d): each size must be a power of two in [16,1024]", nx, ny, nz), SCL_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     (void)hipGetLastError();

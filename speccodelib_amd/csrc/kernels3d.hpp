@@ -151,7 +151,9 @@ struct LinIdx {
 #ifndef SCL_COL_CW
 #define SCL_COL_CW 8
 #endif
-template <class R, int N, int CWv = SCL_COL_CW, int EV = 0>
+// FP64 1024-point lines take 4 columns: 8 would need 163 KB of LDS with the
+// line's twiddles.
+template <class R, int N, int CWv = (sizeof(R) == 8 && N >= 1024 ? 4 : SCL_COL_CW), int EV = 0>
 struct ColCfg {
   static constexpr int CW = CWv;
   static constexpr int E = EV ? EV : (N <= 512 ? 8 : 16);

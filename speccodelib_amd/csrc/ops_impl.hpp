@@ -11,10 +11,11 @@ namespace scl {
 template <int V>
 using IC = std::integral_constant<int, V>;
 
-// Column/line lengths supported per precision.
+// Column/line lengths supported per precision (FP64 1024-point lines run
+// on 4-column tiles: ColCfg).
 template <class R>
 constexpr int max_line() {
-  return sizeof(R) == 4 ? 1024 : 512;
+  return 1024;
 }
 
 template <class R, class F>
@@ -172,7 +173,7 @@ int32_t Ops<R>::zinv_vort(const scl_plan3d& pl, const C* s, int64_t sstride, C* 
     if constexpr (sizeof(R) == 4 && N <= 512) {
       if (zcol16<R>(g)) return go(IC<16>{});
     }
-    return go(IC<8>{});
+    return go(IC<ColCfg<R, N>::CW>{});
   });
 }
 
@@ -227,7 +228,7 @@ int32_t Ops<R>::zfwd_rhs(const scl_plan3d& pl, int mode, C* W, int64_t fstride, 
     if constexpr (sizeof(R) == 4 && N <= 512) {
       if (zcol16<R>(g)) return go(IC<16>{});
     }
-    return go(IC<8>{});
+    return go(IC<ColCfg<R, N>::CW>{});
   });
 }
 

@@ -14,7 +14,7 @@ grids the FP64 oracle finishes in seconds:
 Inputs are white-noise spectra (Hermitian on kx = 0), so every wavenumber
 is exercised.  Bars: 2e-5 of the largest entry (FP32 rounding times log2 of
 the longest line) for RHS and short runs against the FP64 oracle; FP64 plans
-(512-point rows) 1e-11."""
+(512- and 1024-point rows) 1e-11."""
 import numpy as np
 import pytest
 
@@ -46,7 +46,8 @@ def white(shape, seed, k2, nf=3):
 
 
 @pytest.mark.parametrize("dims,prec", [((256, 16, 16), scl.SP), ((512, 16, 16), scl.SP), ((1024, 16, 16), scl.SP),
-                                       ((512, 32, 16), scl.SP), ((512, 16, 16), scl.DP), ((512, 32, 16), scl.DP)])
+                                       ((512, 32, 16), scl.SP), ((512, 16, 16), scl.DP), ((512, 32, 16), scl.DP),
+                                       ((1024, 16, 16), scl.DP)])
 def test_dns_wide_rows(dims, prec):
     nx, ny, nz = dims
     bar = BAR if prec == scl.SP else BAR_DP

@@ -1,8 +1,8 @@
 """GPU parity of the long z-line kernels against the oracle: grids whose
 z-lines have 512 and 1024 points (the 512^3 and 1024^3 runs' z-transforms,
 in 1024-thread workgroups that keep one transformed component in LDS) on
-small x-y planes, so the FP64 oracle finishes in seconds; FP64 plans (which
-go up to 512 points per axis) on 512-point z- and y-lines.
+small x-y planes, so the FP64 oracle finishes in seconds; FP64 plans (four-
+column tiles at 1024 points) on 512- and 1024-point z- and y-lines.
 
 Inputs are white-noise spectra over every mode (Hermitian on kx = 0), so
 every z-wavenumber is exercised.  ratesdudt (dns-deform.f90:259-288) runs
@@ -18,7 +18,8 @@ from oracle import spectral3d as orc
 
 pytestmark = pytest.mark.gpu
 
-CASES = [((32, 32, 512), scl.SP), ((32, 16, 1024), scl.SP), ((32, 32, 512), scl.DP), ((16, 512, 16), scl.DP)]
+CASES = [((32, 32, 512), scl.SP), ((32, 16, 1024), scl.SP), ((32, 32, 512), scl.DP), ((16, 512, 16), scl.DP),
+         ((32, 16, 1024), scl.DP), ((16, 1024, 16), scl.DP)]
 
 
 def relerr(a, b):
