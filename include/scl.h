@@ -348,7 +348,7 @@ void scl_derivs_(const double *x, const double *y, double *dydx, const int32_t *
  * rate = c_hyper k^n_hyper + c_hypo k^-n_hypo + damp (the reference's
  * (k sigma)^nhyper + (nu/k)^nhypo is c_hyper = sigma^nhyper,
  * c_hypo = nu^nhypo, damp = 0), in place of the reference's IF-AB2
- * (:192-219).  Sizes: powers of two in [32, 1024] (FP64: [32, 256]).
+ * (:192-219).  Sizes: powers of two in [32, 1024] (FP32 and FP64).
  *
  * per_member = 0: every size group advances as one batch (one set of
  * launches per group, the groups on parallel branches of one hipGraph per

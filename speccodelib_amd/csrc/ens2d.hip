@@ -46,10 +46,11 @@ struct Group {
 // nonlinear term in a seventh field, F_NA.
 enum { F_W = 0, F_WS = 3, F_ACC = 4, F_S = 5, F_NA = 6, NFIELDS = 6, NFIELDS_TURB = 7 };
 
-// FP64 plans keep one schedule (E = 8, 8 columns) and lines up to 256.
+// FP64 plans keep one schedule: E = 8 with 8 columns, 4 at 1024 points
+// (8 would need 163 KB of LDS with the line's twiddles).
 template <class R, int N, int E, int CW>
 constexpr bool schedule_ok() {
-  if constexpr (sizeof(R) == 8) return N <= 256 && E == 8 && CW == 8 && Col2D<R, N, E, CW>::valid;
+  if constexpr (sizeof(R) == 8) return E == 8 && CW == (N >= 1024 ? 4 : 8) && Col2D<R, N, E, CW>::valid;
   else return Col2D<R, N, E, CW>::valid;
 }
 
@@ -96,15 +97,11 @@ int32_t with_row2d(int m, F&& f) {
     case 32: return f(Int<32>{});
     case 64: return f(Int<64>{});
     case 128: return f(Int<128>{});
-    case 256:
-      if constexpr (sizeof(R) == 4) return f(Int<256>{});
-      break;
-    case 512:
-      if constexpr (sizeof(R) == 4) return f(Int<512>{});
-      break;
+    case 256: return f(Int<256>{});
+    case 512: return f(Int<512>{});
     default: break;
   }
-  set_error("2D size %d is not available in this precision", 2 * m);
+  set_error("2D size %d is not available", 2 * m);
   return SCL_EINVAL;
 }
 
@@ -535,12 +532,11 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
   *out = nullptr;
   if (prec != SCL_SP && prec != SCL_DP) return set_error("bad precision %d", prec), SCL_EINVAL;
   if (ngroups < 1) return set_error("ngroups must be >= 1"), SCL_EINVAL;
-  const int hi = prec == SCL_SP ? 1024 : 256;
   for (int g = 0; g < ngroups; ++g) {
     const int n = sizes[g];
     // n >= 32: the x-pass packs 1024/n rows per workgroup and needs n^2 >= 1024
-    if (n < 32 || n > hi || (n & (n - 1)))
-      return set_error("group %d: size %d must be a power of two in [32, %d]", g, n, hi), SCL_EINVAL;
+    if (n < 32 || n > 1024 || (n & (n - 1)))
+      return set_error("group %d: size %d must be a power of two in [32, 1024]", g, n), SCL_EINVAL;
     if (members[g] < 1) return set_error("group %d: members must be >= 1", g), SCL_EINVAL;
   }
   if (!(params->dt > 0)) return set_error("dt must be > 0"), SCL_EINVAL;
@@ -600,7 +596,7 @@ extern "C" int32_t scl_ens2d_create(int32_t prec, int32_t ngroups, const int32_t
       gr.g = make_geom(n, n, nb);
       gr.tw = tw;
       gr.e = 8;  // at 1024 points: one 1024-thread workgroup per CU, 4 waves per SIMD
-      gr.cw = 8;
+      gr.cw = prec == SCL_DP && n >= 1024 ? 4 : 8;  // FP64 1024: 4 columns (LDS)
       const size_t bytes = (size_t)(params->scheme == 1 ? NFIELDS_TURB : NFIELDS) * gr.fs() * es;
       if (hipMalloc(&gr.base, bytes) != hipSuccess) {
         (void)hipGetLastError();

@@ -12,7 +12,13 @@ Bars: the velocity fields within 1e-5 of their largest entry, the energy,
 dissipation and enstrophy (and at 1024^3 the helicity, on the scale
 max(|H|, sqrt(E Omega))) within 1e-5 relative, E(k) within 1e-4 relative on every shell holding more than 1e-12
 of the total (FP32 rounding, 6e-8, over tens of steps of a smooth,
-well-resolved flow: about 1e-6 is expected)."""
+well-resolved flow: about 1e-6 is expected).
+
+The 2D ensembles bench.py times (192 x 256^2 + 48 x 512^2 + 16 x 1024^2
+members, spectral_dns_2d.f90's RHS with IF-RK4, the same seeded start)
+likewise after 50 batched steps: every member's energy and enstrophy
+within 1e-5 relative, the first and last member of each size within 1e-4
+of their largest entry."""
 import numpy as np
 import pytest
 
@@ -67,3 +73,29 @@ def test_512_fp32_against_fp64_plan():
 
 def test_1024_helical_fp32_against_fp64_plan():
     _compare(1024, 10, HELICAL, ("energy", "dissipation", "enstrophy", "helicity"))
+
+
+ENS_GROUPS = [(256, 192), (512, 48), (1024, 16)]
+ENS_KW = dict(dt=5e-4, c_hyper=1e-5, n_hyper=8, damp=0.1, fkmin=4.0, fkmax=5.0, eps=0.01)
+
+
+def test_ensemble_2d_fp32_against_fp64():
+    print("case 2D ensembles 192x256^2 + 48x512^2 + 16x1024^2, 50 steps, FP32 vs FP64", flush=True)
+    out = {}
+    for prec in (scl.SP, scl.DP):
+        e = scl.Ensemble2D(ENS_GROUPS, prec=prec, **ENS_KW)
+        e.initialize(seed0=-1, k0=8.0, energy=0.5)
+        e.step(50)
+        e.sync()
+        fields = {(g, j): e.get_member(g, j) for g, (_, m) in enumerate(ENS_GROUPS) for j in (0, m - 1)}
+        out[prec] = (e.diagnostics(), fields)
+        e.close()
+        print(f"prec {prec} done", flush=True)
+    (d32, f32), (d64, f64) = out[scl.SP], out[scl.DP]
+    r = np.abs(d32 - d64) / np.abs(d64)
+    print(f"member energy and enstrophy: max rel {r.max():.2e}", flush=True)
+    assert r.max() < 1e-5
+    for key, b in f64.items():
+        err = np.abs(f32[key].astype(np.complex128) - b).max() / np.abs(b).max()
+        print(f"group {key[0]} member {key[1]}: rel err {err:.2e}", flush=True)
+        assert err < 1e-4
