@@ -48,10 +48,8 @@ def ens(groups, prec, dt=1e-3, n=None, **kw):
 
 
 @pytest.mark.parametrize("prec,tol", [(scl.DP, 1e-12), (scl.SP, 2e-5)])
-@pytest.mark.parametrize("n", [32, 64, 256])
+@pytest.mark.parametrize("n", [32, 64, 256, 512])
 def test_nonlinear_term(prec, tol, n):
-    if prec == scl.DP and n > 256:
-        pytest.skip("FP64 ensembles go to 256")
     print(f"case turb2d rhs n={n} prec={prec}", flush=True)
     w = field(n, n)
     e = ens([(n, 1)], prec)
