@@ -374,19 +374,6 @@ __device__ inline cpx<R> stage_apply(const StageIn<R>& in, cpx<R> a, I o, cpx<R>
   return out;
 }
 
-template <int MODE, class R, class I, bool NT = false>
-__device__ inline cpx<R> stage_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
-                                    bool store_s, unsigned* nf = nullptr) {
-  return stage_apply<MODE, R>(stage_load<MODE, R, I, NT>(o, V, DU, S, cut), a, o, V, DU, S, em1, x, cut, store_s,
-                              nf);
-}
-
-template <int MODE, class R, class I>
-__device__ inline void stage_update(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R em1, R x, bool cut,
-                                    unsigned* nf = nullptr) {
-  stage_keep<MODE, R>(a, o, V, DU, S, em1, x, cut, true, nf);
-}
-
 // Parameters of the fused ZFWD epilogue.
 template <class R>
 struct RhsParams {
@@ -412,7 +399,7 @@ enum { AB_TIME = 0, AB_DT = 1, AB_DTO = 2, AB_UMAX = 3, AB_TWRITE = 4, AB_ISTEP 
 //   MODE 5 (step 0):  v1 = (v0 + dt a) E(dt);  DU = v0, S = a (N0)
 //   MODE 6 (step 1):  v2 = [cS E(dto) v0 + cek v1 + cdel a] E(dt)  (v0 from DU)
 //   MODE 7 (>= 2):    v = [v + cS a + cek E(dto) O] E(dt);  S = a (O = S)
-// The new v is 0 where cut (k2 >= kcut^2, and the mean).  nf: as stage_keep.
+// The new v is 0 where cut (k2 >= kcut^2, and the mean).  in and nf: as stage_apply.
 template <int MODE, class R, class I>
 __device__ inline cpx<R> ab_apply(const StageIn<R>& in, cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS,
                                   R cek, R cdel, R e1, R e2, bool cut, unsigned* nf) {
@@ -436,12 +423,6 @@ __device__ inline cpx<R> ab_apply(const StageIn<R>& in, cpx<R> a, I o, cpx<R>* V
   return out;
 }
 
-template <int MODE, class R, class I, bool NT = false>
-__device__ inline cpx<R> ab_keep(cpx<R> a, I o, cpx<R>* V, cpx<R>* DU, cpx<R>* S, R dt, R cS, R cek, R cdel, R e1,
-                                 R e2, bool cut, unsigned* nf) {
-  return ab_apply<MODE, R>(stage_load<MODE, R, I, NT>(o, V, DU, S, cut), a, o, V, DU, S, dt, cS, cek, cdel, e1, e2,
-                           cut, nf);
-}
 
 // ZFWD: z-forward of the three transformed products + forcing + the
 // pointwise part of symmetrize + projection (+ optional RHS truncation),
@@ -517,7 +498,8 @@ __global__ __launch_bounds__((ColCfg<R, N, CWZ, EZ>::BS)) void k_zfwd_rhs(cpx<R>
     const int z = L::out_pos(m, rt);
     const int iy = kwave(t.oth, g.ny), iz = kwave(z, g.nz);
     const int ik2 = t.kx * t.kx + iy * iy + iz * iz;
-    const bool cut = (ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2) >= p.kc2;  // what MODE 4 reads by
+    // MODE 4 reads du only inside the cut
+    const bool cut = (ik2 == 0 ? std::numeric_limits<R>::min() : (R)ik2) >= p.kc2;
     const uint32_t o = zz.at(z);
     dst[0] = stage_load<MODE, R, uint32_t, NT>(o, V, DU, S, cut);
     dst[1] = stage_load<MODE, R, uint32_t, NT>(o, V + fstride, DU + fstride, S + fstride, cut);

@@ -248,7 +248,7 @@ struct ExtParams {
 };
 
 // The update of one component in the LES / scalar combine: the IF-RK4 stage
-// MODE 1..4 (stage_update), or the IF-AB2 step of MODE 5..7 (ab_keep, the
+// MODE 1..4 (stage_apply), or the IF-AB2 step of MODE 5..7 (ab_apply, the
 // weights this step's k_ab_begin left in r.ab) as spectral_les.f90:241-304
 // and spectral_dns_scalar.f90:257-338 write it, with the component's
 // viscosity nu (nu for u, nu/Sc for the scalar); AB2 also zeroes the mean.
