@@ -93,6 +93,14 @@ def cpu_baseline(n, steps, scl):
     for _ in range(steps):
         v = d.rk4_step(*v)
     dt = time.perf_counter() - t0
+    # one thread, on the same configuration at half the size (a bounded sample)
+    n1 = max(16, n // 2)
+    orc.set_fft_backend("scipy", workers=1)
+    v1 = scl.initialize(n1, n1, n1, 1, 4.0, 1.0, scl.SP)
+    d1 = orc.DNS(orc.DNSParams(nx=n1, nu=0.01, dt=0.002, forced=True, fkmax=2.0, eps=0.1, prec="sp"))
+    t0 = time.perf_counter()
+    d1.rk4_step(*v1)
+    dt1 = time.perf_counter() - t0
     return {
         "value": steps / dt,
         "unit": "steps/s",
@@ -101,6 +109,14 @@ def cpu_baseline(n, steps, scl):
         "kind": "port",
         "sample": f"{steps} IF-RK4 step(s) of the same {n}^3 FP32 forced DNS, oracle (numpy + scipy.fft "
         f"workers={cores}), {dt:.1f} s",
+        "ns_per_point_step": dt * 1e9 / (steps * n**3),
+        "one_thread": {
+            "value": 1.0 / dt1,
+            "unit": "steps/s",
+            "ns_per_point_step": dt1 * 1e9 / n1**3,
+            "sample": f"1 IF-RK4 step of the same forced DNS at {n1}^3, oracle with scipy.fft workers=1, "
+            f"{dt1:.1f} s",
+        },
     }
 
 
