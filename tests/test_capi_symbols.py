@@ -32,3 +32,20 @@ def test_errors_are_reported_without_a_plan():
     st = lib.scl_symmetrize3d(None, None, 9, 16, 16)
     assert st == -4  # SCL_ENOPLAN
     assert b"fftwplan3d" in lib.scl_last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(scl.LIBPATH), reason="libscl.so not built")
+@pytest.mark.parametrize("prec", [scl.SP, scl.DP])
+def test_plan_sizes_checked_before_any_device_call(prec):
+    """Both precisions take powers of two in [16, 1024] per axis; a size
+    outside is SCL_EINVAL before the device is touched (without a GPU an
+    accepted size then reports SCL_ENODEV)."""
+    lib = scl.lib()
+    h = ctypes.c_void_p()
+    assert lib.scl_plan3d_create(2048, 16, 16, prec, ctypes.byref(h)) == -1
+    assert b"[16,1024]" in lib.scl_last_error()
+    assert lib.scl_plan3d_create(1024, 16, 24, prec, ctypes.byref(h)) == -1
+    st = lib.scl_plan3d_create(1024, 16, 16, prec, ctypes.byref(h))
+    assert st in (0, -6), st
+    if st == 0:
+        lib.scl_plan3d_destroy(h)
